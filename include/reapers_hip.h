@@ -1,0 +1,167 @@
+/*
+ * reapers_hip.h -- C ABI of the gfx950 backend (libreapers_hip.so).
+ *
+ * This is the drop-in boundary: the header-only C++ API (reapers_amd/include/
+ * reapers/) implements the reference's GPUImpl static interface on top of
+ * these functions, and no C++ or torch type crosses it.  Every entry point
+ * returns an int status (RP_OK on success); rp_last_error() describes the
+ * last failure of the calling thread.  The C++ layer maps the status codes to
+ * the reference's exception classes (inc/except.h:100-170).
+ *
+ * Conventions (the reference's, inc/gpuimpl.h:396-1048):
+ *   - one device, one library stream with default priority; vector kernels
+ *     are ordered on that stream, every function that returns a value to the
+ *     host synchronizes first, rp_synchronize() waits for all queued work;
+ *   - device pointers are owned by the caller (rp_malloc/rp_free); a term
+ *     handle (rp_terms_t) is owned by the library until rp_terms_free();
+ *   - complex vectors are interleaved (re, im): RP_C64 = complex<float>
+ *     (8 bytes), RP_C128 = complex<double> (16 bytes);
+ *   - n is the number of complex elements; no power-of-two restriction.
+ *
+ * Each declaration names the reference interface it replaces (file:line
+ * under the reference tree).
+ */
+#ifndef REAPERS_HIP_H
+#define REAPERS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes; the C++ layer throws the reference's exception for each */
+#define RP_OK                   0
+#define RP_ERR_INVALID_ARGUMENT 1   /* InvalidArgument (inc/except.h:100-117) */
+#define RP_ERR_OUT_OF_MEMORY    2   /* DevOutOfMem (inc/except.h:160-170)     */
+#define RP_ERR_RUNTIME          3   /* CudaError/RuntimeError (inc/except.h:119-141) */
+#define RP_ERR_NO_DEVICE        4   /* NoGpuDevice (inc/except.h:143-149)     */
+#define RP_ERR_NOT_INITIALIZED  5
+
+/* element precisions */
+#define RP_C64  0                   /* complex<float>  */
+#define RP_C128 1                   /* complex<double> */
+
+/* rounding of a complex<double> -> complex<float> conversion */
+#define RP_ROUND_NEAREST      0     /* CPUImpl::copy_vec (inc/cpuimpl.h:620-629) */
+#define RP_ROUND_TOWARD_ZERO  1     /* krnl_dtof_vec (src/krnl.cu:33-39)         */
+
+/* memcpy directions */
+#define RP_H2D 0
+#define RP_D2H 1
+#define RP_D2D 2
+
+/* ---------------------------------------------------------------------- */
+/* lifecycle -- GPUContext (inc/gpuimpl.h:496-557), sync_devices          */
+/* (inc/gpuimpl.h:628-638)                                                 */
+/* ---------------------------------------------------------------------- */
+int         rp_init(int device);
+int         rp_shutdown(void);
+const char *rp_last_error(void);
+int         rp_synchronize(void);
+/* cudaMemGetInfo in GPUContext (inc/gpuimpl.h:553-556) */
+int         rp_mem_get_info(size_t *free_bytes, size_t *total_bytes);
+/* number of compute units and the device name, for launch sizing/reporting */
+int         rp_device_props(int *num_cus, char *name, size_t name_len);
+
+/* ---------------------------------------------------------------------- */
+/* memory -- DeviceVec (inc/gpuimpl.h:404-494), cuda_alloc               */
+/* (inc/gpuimpl.h:62-66), DeviceVec host<->device conversions              */
+/* (inc/gpuimpl.h:438-462)                                                 */
+/* ---------------------------------------------------------------------- */
+int rp_malloc(void **ptr, size_t bytes);
+int rp_free(void *ptr);
+int rp_memcpy(void *dst, const void *src, size_t bytes, int direction);
+
+/* ---------------------------------------------------------------------- */
+/* BLAS-1 -- GPUImpl vector operations                                     */
+/* ---------------------------------------------------------------------- */
+/* zero_vec (inc/gpuimpl.h:855-862) */
+int rp_zero_vec(int prec, void *v, uint64_t n);
+/* copy_vec, same or converting precision (inc/gpuimpl.h:869-889,
+ * src/krnl.cu:25-39, src/krnl.cu:157-184); rounding applies to C128->C64 */
+int rp_copy_vec(int dst_prec, void *dst, int src_prec, const void *src, uint64_t n,
+                int rounding);
+/* add_vec(v0, v1): v0 += v1 (src/krnl.cu:41-48, inc/gpuimpl.h:908-918) */
+int rp_add_vec(int prec, void *v0, const void *v1, uint64_t n);
+/* add_vec(res, v0, v1): res = v0 + v1 (src/krnl.cu:50-58, inc/gpuimpl.h:944-955) */
+int rp_add_vec3(int prec, void *res, const void *v0, const void *v1, uint64_t n);
+/* add_vec(v0, s, v1): v0 += s*v1, s complex (inc/gpuimpl.h:806-818,
+ * inc/gpuimpl.h:920-942; cublasXaxpy) */
+int rp_axpy(int prec, void *v0, double s_re, double s_im, const void *v1, uint64_t n);
+/* scale_vec(v, s): v *= s (inc/gpuimpl.h:822-830, inc/gpuimpl.h:957-976) */
+int rp_scale_vec(int prec, void *v, double s_re, double s_im, uint64_t n);
+/* vec_prod: <v0|v1> = sum conj(v0[i]) v1[i] (inc/gpuimpl.h:980-995;
+ * cublasXdotc), accumulated in FP64 for both precisions */
+int rp_vec_prod(int prec, const void *v0, const void *v1, uint64_t n, double *re, double *im);
+/* vec_norm = sqrt(Re <v|v>) (inc/gpuimpl.h:999-1002), FP64 accumulation */
+int rp_vec_norm(int prec, const void *v, uint64_t n, double *nrm);
+/* init_random (inc/gpuimpl.h:1050-1062): Re, Im ~ N(0,1) from a seeded
+ * Philox4x32-10 stream (the reference seeds cuRAND from random_device), then
+ * normalized with an FP64 norm */
+int rp_init_random(int prec, void *v, uint64_t n, uint64_t seed);
+
+/* ---------------------------------------------------------------------- */
+/* Pauli-sum operator -- DevSumOps (inc/gpuimpl.h:125-393)                 */
+/* ---------------------------------------------------------------------- */
+typedef struct rp_terms *rp_terms_t;
+
+/* DevSumOps::upload (inc/gpuimpl.h:1064-1085): takes the host term list of a
+ * SumOps -- SpinOp bits (2 bits per site, X at bit 2s, Z at bit 2s+1,
+ * inc/ops.h:133-143) and the coefficient (the SpinOp coeff member, in which
+ * sigma_y carries its i: inc/ops.h:285-288) as interleaved doubles -- and
+ * builds the flip/phase masks and the grouped device layout the apply kernel
+ * uses.  len is the spin chain length (1..32). */
+int rp_terms_upload(rp_terms_t *out, int prec, int len, const uint64_t *bits,
+                    const double *coeff, uint64_t nterms);
+/* DevSumOps::release (inc/gpuimpl.h:1087-1105) */
+int rp_terms_free(rp_terms_t h);
+
+typedef struct {
+    int      prec;          /* RP_C64 / RP_C128 */
+    int      len;           /* spin chain length */
+    int      tile_bits;     /* b: log2 of the amplitude tile one workgroup owns */
+    uint64_t nterms;        /* terms after dropping zero coefficients */
+    uint64_t nflips;        /* distinct flip masks x (G_0) */
+    uint64_t ngroups;       /* distinct x >> b: input tile passes (G_b) */
+    uint64_t bytes_per_apply; /* algorithmic HBM bytes (G_b + 1) * 2^len * S */
+} rp_terms_info_t;
+int rp_terms_info(rp_terms_t h, rp_terms_info_t *info);
+
+/* GPUImpl::apply_ops + krnl_apply_ops (src/krnl.cu:220-257, 80-94):
+ *   accumulate != 0: res += H vec   (the reference's semantics)
+ *   accumulate == 0: res  = H vec   (zero_vec fused into the apply)
+ * res and vec must be different buffers of 2^len elements of the handle's
+ * precision. */
+int rp_apply_ops(rp_terms_t h, void *res, const void *vec, int accumulate);
+
+/* ---------------------------------------------------------------------- */
+/* fused Krylov sweeps -- replace the per-vector dot/axpy/norm calls of    */
+/* Krylov::orthogonalize (inc/algo.h:76-107) and the answer update of      */
+/* Krylov::evolve (inc/algo.h:336-345)                                     */
+/* ---------------------------------------------------------------------- */
+/* One classical Gram-Schmidt pass of w against V[0..k-1] (device pointers):
+ *   h[i] = <V[i]|w> (FP64), w -= sum_i h[i] V[i], *norm = ||w|| (FP64).
+ * h receives 2k doubles (re, im).  k may be 0 (then only the norm). */
+int rp_gs_pass(int prec, const void *const *V, int k, void *w, uint64_t n,
+               double *h, double *norm);
+/* y += sum_i c[i] V[i] for k vectors, c given as 2k doubles (re, im) */
+int rp_lincomb(int prec, void *y, const void *const *V, int k, const double *c, uint64_t n);
+
+/* ---------------------------------------------------------------------- */
+/* kernel timing (HIP events on the library stream)                       */
+/* ---------------------------------------------------------------------- */
+#define RP_KERNEL_APPLY 0           /* rp_apply_ops launches  */
+#define RP_KERNEL_GS    1           /* rp_gs_pass launches    */
+#define RP_KERNEL_OTHER 2           /* every other launch     */
+/* when enabled, each launch is bracketed by events and its duration added to
+ * a per-class total at the next synchronization */
+int rp_prof_enable(int on);
+int rp_prof_reset(void);
+int rp_prof_query(int kernel_class, uint64_t *launches, double *total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

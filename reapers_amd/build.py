@@ -1,0 +1,109 @@
+"""Build recipes for the native parts of reapers_amd.
+
+- libreapers_hip.so  : the gfx950 HIP kernels behind the C ABI in
+                       include/reapers_hip.h (hipcc, one object per .hip file)
+- libreapers_host.so : the header-only C++ API instantiated for Python callers
+                       (g++, links libreapers_hip.so)
+
+Run ``python -m reapers_amd.build`` or call :func:`build_all`.
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+
+HIP_SOURCES = ["rp_runtime.hip", "rp_vec.hip", "rp_apply.hip"]
+HOST_SOURCES = ["host_api.cc"]
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+# -packed-fp32-ops: no v_pk_* FP32 instructions anywhere in the device code
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-result",
+    "-I" + os.path.join(ROOT, "include"),
+]
+CXX = os.environ.get("CXX", "g++")
+HOST_FLAGS = [
+    "-O2", "-fPIC", "-std=c++20", "-fopenmp", "-ffp-contract=off", "-Wall",
+    "-Wno-unused-function",
+    "-I" + os.path.join(ROOT, "include"),
+    "-I" + os.path.join(HERE, "include"),
+]
+
+
+def _run(cmd):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+
+
+def _stale(target, deps):
+    if not os.path.exists(target):
+        return True
+    t = os.path.getmtime(target)
+    return any(os.path.getmtime(d) > t for d in deps)
+
+
+def _headers(dirpath, exts=(".h", ".hpp")):
+    out = []
+    for base, _, files in os.walk(dirpath):
+        for f in files:
+            if f.endswith(exts):
+                out.append(os.path.join(base, f))
+    return out
+
+
+def build_hip(force=False):
+    os.makedirs(BUILD, exist_ok=True)
+    deps = _headers(CSRC) + [os.path.join(ROOT, "include", "reapers_hip.h")]
+    objs = []
+    for src in HIP_SOURCES:
+        s = os.path.join(CSRC, src)
+        o = os.path.join(BUILD, src.replace(".hip", ".o"))
+        if force or _stale(o, [s] + deps):
+            _run([HIPCC] + HIP_FLAGS + ["-c", s, "-o", o])
+        objs.append(o)
+    lib = os.path.join(HERE, "libreapers_hip.so")
+    if force or _stale(lib, objs):
+        tmp = lib + ".tmp"
+        _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
+        os.replace(tmp, lib)
+    return lib
+
+
+def build_host(force=False):
+    deps = (_headers(os.path.join(HERE, "include")) + _headers(CSRC)
+            + [os.path.join(ROOT, "include", "reapers_hip.h"),
+               os.path.join(HERE, "libreapers_hip.so")])
+    srcs = [os.path.join(CSRC, s) for s in HOST_SOURCES]
+    lib = os.path.join(HERE, "libreapers_host.so")
+    if force or _stale(lib, srcs + deps):
+        tmp = lib + ".tmp"
+        _run([CXX] + HOST_FLAGS + ["-shared", "-o", tmp] + srcs
+             + ["-L" + HERE, "-lreapers_hip", "-Wl,-rpath,$ORIGIN",
+                "-Wl,--no-as-needed"])
+        os.replace(tmp, lib)
+    return lib
+
+
+def build_oracle(force=False):
+    odir = os.path.join(ROOT, "oracle")
+    if force:
+        _run(["make", "-C", odir, "-B"])
+    else:
+        _run(["make", "-C", odir])
+    return os.path.join(odir, "liboracle.so")
+
+
+def build_all(force=False):
+    build_hip(force)
+    build_host(force)
+    build_oracle(force)
+
+
+if __name__ == "__main__":
+    build_all(force="--force" in sys.argv)

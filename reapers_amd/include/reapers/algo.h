@@ -1,0 +1,208 @@
+// algo.h -- evolution algorithms on State (included inside
+// namespace EvolutionAlgorithm by state.h, as in the reference).
+//
+// Krylov<Impl>::evolve computes s = exp(-(beta + i t) H) s by the restarted
+// Arnoldi method of Eiermann and Ernst (SIAM J. Numer. Anal. 44 (2006)
+// 2481-2504, Alg. 2), following the reference's Krylov::evolve
+// (inc/algo.h:266-365) step by step: Arnoldi of dimension m from the
+// normalized vector, H appended block-bidiagonally to Hfull with the previous
+// restart's beta at (r*m, r*m-1), Hexp = exp((-beta - i t) Hfull), the answer
+// updated with beta0 * Hexp[(r*m + i), 0] * V_i, convergence when the 2-norm
+// of that column block drops below tol, restart from V_m.
+//
+// Differences that are deliberate:
+//   - Gram-Schmidt: the reference subtracts one projection at a time with a
+//     blocking dot product each (inc/algo.h:86-96); here each pass is one
+//     fused sweep that forms all j+1 coefficients from the same vector
+//     (classical Gram-Schmidt) in FP64 and applies them in a second sweep.
+//     The reference's selective criterion decides whether to repeat a pass
+//     (at most 3, eta = 1/sqrt(2): inc/algo.h:79-101), or, with
+//     reorth = Reorth::Always, every step makes exactly two passes.
+//   - Hfull, H, beta and the norms are FP64 for FP32 states as well.
+//   - the optional sub-step splits t into equal pieces, each a separate
+//     restarted evolution (the reference evolves the whole t at once).
+#pragma once
+
+template<typename Impl>
+class Krylov {
+    template<RealScalar FpType>
+    using SumOps = typename Impl::template SumOps<FpType>;
+    using cd = std::complex<double>;
+
+    // iterated fused Gram-Schmidt of buf(j+1) against buf(0..j)
+    template<RealScalar FpType>
+    static bool orthogonalize(State<FpType, Impl> &s, dense::CMat &H, int j, double &beta,
+                              Reorth mode, KrylovStats &st) {
+        const double eta = 1 / std::sqrt(2.0);
+        const size_t dim = s.dim();
+        std::vector<const typename Impl::template VecType<FpType> *> V(j + 1);
+        for (int i = 0; i <= j; i++) V[i] = &s.buf(i);
+        std::vector<cd> h(j + 1, cd(0)), r;
+        double old_norm = 1.0, norm = eta / 2;
+        int niter = 0;
+        const int max_passes = mode == Reorth::Always ? 2 : 3;
+        while (niter < max_passes && norm != 0 &&
+               (mode == Reorth::Always || norm < eta * old_norm)) {
+            niter++;
+            double nrm = Impl::gs_pass(dim, V, s.buf(j + 1), r);
+            for (int i = 0; i <= j; i++) h[i] += r[i];
+            old_norm = norm;
+            norm = nrm;
+            st.gs_passes++;
+        }
+        beta = norm;
+        for (int i = 0; i <= j; i++) H(i, j) = h[i];
+        return norm != 0 && norm >= eta * old_norm;
+    }
+
+    // m-step Arnoldi from the normalized buf(0) (inc/algo.h:144-173)
+    template<RealScalar FpType>
+    static bool arnoldi(State<FpType, Impl> &s, const SumOps<FpType> &A, dense::CMat &H, int &m,
+                        double &beta, Reorth mode, KrylovStats &st) {
+        const size_t dim = s.dim();
+        const auto len = s.spin_chain_length();
+        for (int j = 0; j < m; j++) {
+            Impl::apply_ops_overwrite(len, s.buf(j + 1), A, s.buf(j));
+            st.matvecs++;
+            bool ok = orthogonalize(s, H, j, beta, mode, st);
+            if (!ok) {
+                m = j + 1;
+                return false;
+            }
+            if (j != m - 1) H(j + 1, j) = beta;
+            Impl::scale_vec(dim, s.buf(j + 1), (FpType)(1.0 / beta));
+        }
+        return true;
+    }
+
+    // one restarted evolution over the whole (t, beta)
+    template<RealScalar FpType>
+    static void evolve_once(State<FpType, Impl> &s, const SumOps<FpType> &ops, double t_real,
+                            double t_imag, const KrylovOptions &opt, KrylovStats &st) {
+        const int krydim = opt.krydim;
+        const double tol = opt.tol > 0 ? opt.tol : (double)epsilon<FpType>();
+        const size_t dim = s.dim();
+        s.enlarge(krydim + 2);
+        Impl::zero_vec(dim, s.buf(krydim + 1));
+        const double beta0 = Impl::vec_norm64(dim, s.buf());
+        Impl::scale_vec(dim, s.buf(), (FpType)(1.0 / beta0));
+        dense::CMat Hfull;
+        int niters = 0;
+        double oldbeta = 0.0;
+        while (true) {
+            niters++;
+            int m = krydim;
+            double beta = 0;
+            dense::CMat H(krydim);
+            bool brokedown = !arnoldi(s, ops, H, m, beta, opt.reorth, st);
+            auto t0 = std::chrono::steady_clock::now();
+            const int off = (niters - 1) * krydim;
+            if (niters == 1) {
+                Hfull = H;
+            } else {
+                Hfull.grow(off + m);
+                Hfull(off, off - 1) = oldbeta;
+                for (int i = 0; i < m; i++)
+                    for (int j = 0; j < m; j++) Hfull(off + i, off + j) = H(i, j);
+            }
+            dense::CMat tH = Hfull;
+            const cd z{-t_imag, -t_real};
+            for (auto &v : tH.a) v *= z;
+            dense::CMat E = dense::expm(tH);
+            std::vector<cd> c(m);
+            double nrm2 = 0;
+            for (int i = 0; i < m; i++) {
+                cd e = E(off + i, 0);
+                c[i] = beta0 * e;
+                nrm2 += std::norm(e);
+            }
+            st.host_ms += std::chrono::duration<double, std::milli>(
+                              std::chrono::steady_clock::now() - t0).count();
+            std::vector<const typename Impl::template VecType<FpType> *> V(m);
+            for (int i = 0; i < m; i++) V[i] = &s.buf(i);
+            Impl::lincomb(dim, s.buf(krydim + 1), V, c);
+            const double nrm = std::sqrt(nrm2);
+            st.last_nrm = nrm;
+            if ((opt.max_iters && niters >= opt.max_iters) || brokedown || nrm < tol) {
+                st.breakdown |= brokedown ? 1 : 0;
+                s.inc_curbuf(krydim + 1);
+                break;
+            }
+            s.swap_bufs(0, krydim);
+            oldbeta = beta;
+        }
+        st.restarts += niters;
+    }
+
+public:
+    // exp(-(t_imag + i t_real) H) s with the reference's argument list
+    // (inc/algo.h:266-269)
+    template<RealScalar FpType>
+    static void evolve(State<FpType, Impl> &s, const SumOps<FpType> &ops, FpType t_real,
+                       FpType t_imag = 0.0, int krydim = 5, FpType tol = epsilon<FpType>(),
+                       int max_iters = 0) {
+        KrylovOptions opt;
+        opt.krydim = krydim;
+        opt.tol = (double)tol;
+        opt.max_iters = max_iters;
+        KrylovStats st;
+        evolve_ex(s, ops, (double)t_real, (double)t_imag, opt, st);
+    }
+
+    template<RealScalar FpType>
+    static void evolve_ex(State<FpType, Impl> &s, const SumOps<FpType> &ops, double t_real,
+                          double t_imag, const KrylovOptions &opt, KrylovStats &st) {
+        if (opt.krydim < 2) ThrowException(InvalidArgument, "krydim", "must be at least two");
+        int nsub = 1;
+        if (opt.substep > 0) {
+            double span = std::max(std::abs(t_real), std::abs(t_imag));
+            nsub = std::max(1, (int)std::ceil(span / opt.substep - 1e-12));
+        }
+        for (int k = 0; k < nsub; k++)
+            evolve_once(s, ops, t_real / nsub, t_imag / nsub, opt, st);
+        st.substeps += nsub;
+    }
+
+    // Lanczos ground state (inc/algo.h:180-257), starting from the state's
+    // current vector (the reference starts from a Haar random state)
+    template<RealScalar FpType>
+    static FpType ground_state(State<FpType, Impl> &s, const SumOps<FpType> &ham,
+                               int krydim = 5, FpType eps = 10 * epsilon<FpType>(),
+                               int maxiter = 0, bool randomize = true) {
+        const size_t dim = s.dim();
+        const auto len = s.spin_chain_length();
+        if (krydim < 2) ThrowException(InvalidArgument, "krydim", "must be at least two");
+        s.enlarge(krydim + 1);
+        if (randomize) s.random_state();
+        std::vector<double> alpha(krydim), beta(krydim > 1 ? krydim - 1 : 1);
+        int numiter = 0;
+        double energy = 0;
+        while (true) {
+            for (int i = 0; i < krydim; i++) {
+                Impl::apply_ops_overwrite(len, s.buf(i + 1), ham, s.buf(i));
+                alpha[i] = Impl::vec_prod64(dim, s.buf(i), s.buf(i + 1)).real();
+                if (i == krydim - 1) break;
+                Impl::add_vec(dim, s.buf(i + 1), (FpType)(-alpha[i]), s.buf(i));
+                if (i != 0) Impl::add_vec(dim, s.buf(i + 1), (FpType)(-beta[i - 1]), s.buf(i - 1));
+                beta[i] = Impl::vec_norm64(dim, s.buf(i + 1));
+                Impl::scale_vec(dim, s.buf(i + 1), (FpType)(1.0 / beta[i]));
+            }
+            std::vector<double> w, Q;
+            dense::tridiag_eigen(alpha, beta, w, Q);
+            int idx = 0;
+            for (int i = 1; i < krydim; i++) if (w[i] < w[idx]) idx = i;
+            energy = w[idx];
+            std::vector<cd> c(krydim);
+            for (int i = 0; i < krydim; i++) c[i] = Q[(size_t)idx * krydim + i];
+            std::vector<const typename Impl::template VecType<FpType> *> V(krydim);
+            for (int i = 0; i < krydim; i++) V[i] = &s.buf(i);
+            Impl::zero_vec(dim, s.buf(krydim));
+            Impl::lincomb(dim, s.buf(krydim), V, c);
+            s.inc_curbuf(krydim);
+            s.normalize();
+            numiter++;
+            if (!(beta[0] > (double)eps && (!maxiter || numiter < maxiter))) break;
+        }
+        return (FpType)energy;
+    }
+};

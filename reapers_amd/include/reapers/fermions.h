@@ -1,0 +1,149 @@
+// fermions.h -- Majorana operators as Jordan-Wigner Pauli strings.
+//
+// SubspaceView is a SumOps with a fixed spin chain length (the reference's
+// SubspaceView, inc/fermions.h:39-205).  FermionOpNonBlockForm(N, n) is the
+// reference's chi_n on L = N/2 sites (inc/fermions.h:254-306):
+//
+//   chi_n = {sigma_y if n odd, sigma_z if n even}((n-1)/2)
+//           . prod_{(n+1)/2 <= i < N/2} sigma_x(i)       (/ sqrt(2) if standard)
+//
+// so its flip mask covers the sites [n/2, N/2) and its phase mask is the
+// single site (n-1)/2 (none for n = 0).  It is built by the same sequence of
+// products as the reference, so coefficients agree bit for bit.
+#pragma once
+
+#ifndef _REAPERS_H_
+#error "Do not include this file directly. Include the master header file reapers.h"
+#endif
+
+template<typename SumOpsTy>
+class SubspaceView : public SumOpsTy {
+    template<typename S1> friend class SubspaceView;
+    int len;
+
+    void ensure_equal_len(const SubspaceView &rhs) const {
+        if (this->empty() || rhs.empty()) return;
+        if (len != rhs.len) ThrowException(InvalidArgument, "spin chain length", "must be equal");
+    }
+
+public:
+    SubspaceView() : len{} {}
+    template<typename S1>
+    SubspaceView(int len, S1 &&ops) : SumOpsTy(std::forward<S1>(ops)), len(len) {}
+    SubspaceView(const SubspaceView &) = default;
+    SubspaceView(SubspaceView &&) = default;
+    SubspaceView &operator=(const SubspaceView &) = default;
+    SubspaceView &operator=(SubspaceView &&) = default;
+
+    static auto identity(int len) {
+        return SubspaceView<SumOpsTy>(len, SpinOp<typename SumOpsTy::RealScalarType>::identity());
+    }
+
+    using SumOpsTy::operator+=;
+    using SumOpsTy::operator-=;
+    using SumOpsTy::operator*=;
+    using SumOpsTy::operator/=;
+    using SumOpsTy::operator==;
+
+    // operand orders as in the reference (inc/fermions.h:87-175); every
+    // product is HostSumOps::operator* on the underlying term lists
+    SubspaceView operator+(const SumOpsTy &rhs) const {
+        return SubspaceView(len, SumOpsTy::operator+(rhs));
+    }
+
+    template<typename OpTy>
+    friend SubspaceView operator+(OpTy &&lhs, const SubspaceView &rhs) {
+        return SubspaceView{rhs.len, std::forward<OpTy>(lhs) + static_cast<const SumOpsTy &>(rhs)};
+    }
+
+    SubspaceView operator+(const SubspaceView &rhs) const {
+        ensure_equal_len(rhs);
+        if (len) return *this + static_cast<const SumOpsTy &>(rhs);
+        return rhs;
+    }
+
+    SubspaceView &operator+=(const SubspaceView &rhs) {
+        ensure_equal_len(rhs);
+        if (len) SumOpsTy::operator+=(static_cast<const SumOpsTy &>(rhs));
+        else *this = rhs;
+        return *this;
+    }
+
+    SubspaceView operator-(const SumOpsTy &rhs) const {
+        return SubspaceView(len, SumOpsTy::operator-(rhs));
+    }
+
+    SubspaceView operator-(const SubspaceView &rhs) const {
+        ensure_equal_len(rhs);
+        return SubspaceView(len ? len : rhs.len, SumOpsTy::operator-(static_cast<const SumOpsTy &>(rhs)));
+    }
+
+    SubspaceView &operator-=(const SubspaceView &rhs) {
+        ensure_equal_len(rhs);
+        if (!len) len = rhs.len;
+        SumOpsTy::operator-=(static_cast<const SumOpsTy &>(rhs));
+        return *this;
+    }
+
+    SubspaceView operator*(const SumOpsTy &rhs) const {
+        return SubspaceView(len, SumOpsTy::operator*(rhs));
+    }
+
+    template<typename OpTy>
+    friend SubspaceView operator*(OpTy &&lhs, const SubspaceView &rhs) {
+        return SubspaceView{rhs.len, std::forward<OpTy>(lhs) * static_cast<const SumOpsTy &>(rhs)};
+    }
+
+    SubspaceView operator*(const SubspaceView &rhs) const {
+        ensure_equal_len(rhs);
+        return static_cast<const SumOpsTy &>(*this) * rhs;
+    }
+
+    SubspaceView &operator*=(const SubspaceView &rhs) {
+        ensure_equal_len(rhs);
+        SumOpsTy::operator*=(static_cast<const SumOpsTy &>(rhs));
+        return *this;
+    }
+
+    bool operator==(const SubspaceView &rhs) const {
+        if (len != rhs.len) return false;
+        return SumOpsTy::operator==(static_cast<const SumOpsTy &>(rhs));
+    }
+
+    int spin_chain_length() const { return len; }
+};
+
+template<RealScalar FpType = DefFpType, typename Impl = DefImpl>
+class FermionOpNonBlockForm : public SubspaceView<typename Impl::template SumOps<FpType>> {
+    using BaseType = SubspaceView<typename Impl::template SumOps<FpType>>;
+    using IndexType = u8;
+    IndexType N, n;
+
+    static void check_field_index(IndexType N, IndexType n) {
+        if (N < 2) ThrowException(InvalidArgument, "N", "must be at least two");
+        if (N & 1) ThrowException(InvalidArgument, "N", "must be even");
+        if (n > N) ThrowException(FieldIndexTooLarge, n, N);
+    }
+
+public:
+    using ProductType = BaseType;
+
+    FermionOpNonBlockForm(IndexType N, IndexType n, bool standard_gamma = true)
+        : BaseType(N / 2, SpinOp<FpType>::identity()), N(N), n(n) {
+        check_field_index(N, n);
+        for (int i = (n + 1) / 2; i < N / 2; i++) *this *= SpinOp<FpType>::sigma_x(i);
+        if (n != 0)
+            *this *= (n & 1) ? SpinOp<FpType>::sigma_y((n - 1) / 2)
+                             : SpinOp<FpType>::sigma_z((n - 1) / 2);
+        if (standard_gamma) *this /= sqrt(2.0);
+    }
+
+    using BaseType::operator*;
+    using BaseType::operator+=;
+    using BaseType::operator*=;
+    using BaseType::operator/=;
+    using BaseType::operator==;
+};
+
+template<RealScalar FpType = DefFpType, typename Impl = DefImpl>
+using FermionOp = FermionOpNonBlockForm<FpType, Impl>;

@@ -1,0 +1,287 @@
+// gpuimpl.h -- GPUImpl: the reference's GPU backend interface
+// (inc/gpuimpl.h:396-1048) implemented on the C ABI of libreapers_hip.so.
+//
+// State<FpType, GPUImpl> and Krylov<GPUImpl> only see the static interface
+// below (VecType, BufType, zero_vec, copy_vec, add_vec, scale_vec, vec_prod,
+// vec_norm, init_random, apply_ops), as in the reference.  Additions that the
+// reference does not have:
+//   - vec_prod64 / vec_norm64 return the FP64-accumulated values;
+//   - gs_pass / lincomb are one fused sweep each (Krylov uses them);
+//   - copy_vec from double to float takes an explicit rounding mode
+//     (default: round to nearest, as CPUImpl; the reference's GPU path
+//     truncates, src/krnl.cu:33-39).
+// Every call is checked and throws the reference's exception types.  There is
+// no CPU fallback: if the HIP library cannot initialize a device, the first
+// use throws NoGpuDevice.
+#pragma once
+
+#ifndef _REAPERS_H_
+#error "Do not include this file directly. Include the master header file reapers.h"
+#endif
+
+class GPUImpl;
+
+namespace internal {
+template<RealScalar FpType> constexpr int rp_prec() {
+    return std::is_same_v<FpType, float> ? RP_C64 : RP_C128;
+}
+}  // namespace internal
+
+// A SumOps whose device layout is uploaded on first use and dropped on every
+// host mutation (DevSumOps::upload/release, inc/gpuimpl.h:1064-1105).
+template<RealScalar FpType = DefFpType>
+class DevSumOps : public HostSumOps<FpType> {
+    friend class GPUImpl;
+    mutable rp_terms_t handle = nullptr;
+    mutable int handle_len = -1;
+
+    void release() const override {
+        if (handle) {
+            rp_terms_free(handle);
+            handle = nullptr;
+            handle_len = -1;
+        }
+    }
+
+public:
+    using HostSumOps<FpType>::HostSumOps;
+    DevSumOps(const HostSumOps<FpType> &ops) : HostSumOps<FpType>(ops) {}
+    DevSumOps(const DevSumOps &ops) : HostSumOps<FpType>(ops) {}
+    DevSumOps(DevSumOps &&ops) : HostSumOps<FpType>(std::move(ops)) {
+        std::swap(handle, ops.handle);
+        std::swap(handle_len, ops.handle_len);
+    }
+    template<RealScalar F1>
+    explicit DevSumOps(const DevSumOps<F1> &ops) : HostSumOps<FpType>(ops) {}
+    DevSumOps &operator=(const DevSumOps &rhs) {
+        HostSumOps<FpType>::operator=(rhs);
+        return *this;
+    }
+    DevSumOps &operator=(DevSumOps &&rhs) {
+        if (this == &rhs) return *this;
+        HostSumOps<FpType>::operator=(static_cast<const HostSumOps<FpType> &>(rhs));
+        return *this;
+    }
+    using HostSumOps<FpType>::operator=;
+    ~DevSumOps() override { release(); }
+
+    // the uploaded layout for chain length len (uploads on first use)
+    rp_terms_t device_terms(int len) const;
+
+    // layout statistics of the uploaded form (groups, bytes per apply)
+    rp_terms_info_t device_info(int len) const {
+        rp_terms_info_t info;
+        RP_CALL(rp_terms_info(device_terms(len), &info));
+        return info;
+    }
+};
+
+class GPUImpl {
+    template<RealScalar F> friend class DevSumOps;
+
+    struct Context {
+        Context() {
+            int dev = 0;
+            if (const char *e = std::getenv("REAPERS_DEVICE")) dev = std::atoi(e);
+            int st = rp_init(dev);
+            if (st != RP_OK) internal::throw_status(__FILE__, __LINE__, __func__, st);
+        }
+    };
+
+public:
+    static void ensure_context() { static Context c; (void)c; }
+
+    // device vector owning 2^len complex elements (DeviceVec,
+    // inc/gpuimpl.h:404-494)
+    template<RealScalar FpType>
+    class DeviceVec {
+        void *ptr = nullptr;
+        size_t dim = 0;
+        friend class GPUImpl;
+        DeviceVec(const DeviceVec &) = delete;
+        DeviceVec &operator=(const DeviceVec &) = delete;
+    public:
+        explicit DeviceVec(size_t n) : dim(n) {
+            ensure_context();
+            int st = rp_malloc(&ptr, n * sizeof(complex<FpType>));
+            if (st != RP_OK)
+                internal::throw_status(__FILE__, __LINE__, __func__, st,
+                                       n * sizeof(complex<FpType>));
+        }
+        DeviceVec(DeviceVec &&v) : ptr(v.ptr), dim(v.dim) { v.ptr = nullptr; v.dim = 0; }
+        DeviceVec &operator=(DeviceVec &&v) {
+            if (this != &v) {
+                if (ptr) rp_free(ptr);
+                ptr = v.ptr; dim = v.dim;
+                v.ptr = nullptr; v.dim = 0;
+            }
+            return *this;
+        }
+        ~DeviceVec() { if (ptr) rp_free(ptr); }
+        DeviceVec &get() { return *this; }
+        const DeviceVec &get() const { return *this; }
+        void *data() { return ptr; }
+        const void *data() const { return ptr; }
+        size_t dimension() const { return dim; }
+        friend void swap(DeviceVec &a, DeviceVec &b) {
+            std::swap(a.ptr, b.ptr);
+            std::swap(a.dim, b.dim);
+        }
+        complex<FpType> operator[](size_t idx) const {
+            complex<FpType> c;
+            RP_CALL(rp_memcpy(&c, (const complex<FpType> *)ptr + idx, sizeof(c), RP_D2H));
+            return c;
+        }
+        // host <-> device transfers of the whole vector
+        void upload(const complex<FpType> *host) {
+            RP_CALL(rp_memcpy(ptr, host, dim * sizeof(complex<FpType>), RP_H2D));
+        }
+        void download(complex<FpType> *host) const {
+            RP_CALL(rp_memcpy(host, ptr, dim * sizeof(complex<FpType>), RP_D2H));
+        }
+    };
+
+    template<RealScalar FpType> using VecType = DeviceVec<FpType>;
+    template<RealScalar FpType> using VecSizeType = size_t;
+    template<RealScalar FpType> using BufType = DeviceVec<FpType> &;
+    template<RealScalar FpType> using ConstBufType = const DeviceVec<FpType> &;
+    template<RealScalar FpType> using SumOps = DevSumOps<FpType>;
+
+    static void synchronize() { RP_CALL(rp_synchronize()); }
+
+    template<RealScalar FpType>
+    static void zero_vec(size_t size, BufType<FpType> v) {
+        RP_CALL(rp_zero_vec(internal::rp_prec<FpType>(), v.ptr, size));
+    }
+
+    // Haar random state from a seeded Philox stream (the reference seeds
+    // cuRAND from std::random_device, inc/gpuimpl.h:540-551)
+    template<RealScalar FpType>
+    static void init_random(size_t size, BufType<FpType> v, u64 seed = 0) {
+        static u64 counter = 0x5eed;
+        if (!seed) seed = ++counter;
+        RP_CALL(rp_init_random(internal::rp_prec<FpType>(), v.ptr, size, seed));
+    }
+
+    template<RealScalar F0, RealScalar F1>
+    static void copy_vec(size_t size, BufType<F0> v0, ConstBufType<F1> v1,
+                         int rounding = RP_ROUND_NEAREST) {
+        RP_CALL(rp_copy_vec(internal::rp_prec<F0>(), v0.ptr, internal::rp_prec<F1>(), v1.ptr,
+                            size, rounding));
+    }
+
+    template<RealScalar FpType>
+    static void add_vec(size_t size, BufType<FpType> v0, ConstBufType<FpType> v1) {
+        RP_CALL(rp_add_vec(internal::rp_prec<FpType>(), v0.ptr, v1.ptr, size));
+    }
+
+    template<RealScalar FpType>
+    static void add_vec(size_t size, BufType<FpType> res, ConstBufType<FpType> v0,
+                        ConstBufType<FpType> v1) {
+        RP_CALL(rp_add_vec3(internal::rp_prec<FpType>(), res.ptr, v0.ptr, v1.ptr, size));
+    }
+
+    template<RealScalar FpType>
+    static void add_vec(size_t size, BufType<FpType> v0, complex<FpType> s,
+                        ConstBufType<FpType> v1) {
+        RP_CALL(rp_axpy(internal::rp_prec<FpType>(), v0.ptr, (double)s.real(), (double)s.imag(),
+                        v1.ptr, size));
+    }
+
+    template<RealScalar FpType>
+    static void add_vec(size_t size, BufType<FpType> v0, FpType s, ConstBufType<FpType> v1) {
+        RP_CALL(rp_axpy(internal::rp_prec<FpType>(), v0.ptr, (double)s, 0.0, v1.ptr, size));
+    }
+
+    template<RealScalar FpType>
+    static void scale_vec(size_t size, BufType<FpType> v, FpType s) {
+        RP_CALL(rp_scale_vec(internal::rp_prec<FpType>(), v.ptr, (double)s, 0.0, size));
+    }
+
+    template<RealScalar FpType>
+    static void scale_vec(size_t size, BufType<FpType> v, complex<FpType> s) {
+        RP_CALL(rp_scale_vec(internal::rp_prec<FpType>(), v.ptr, (double)s.real(),
+                             (double)s.imag(), size));
+    }
+
+    // FP64-accumulated <v0|v1> and ||v|| for either element precision
+    template<RealScalar FpType>
+    static std::complex<double> vec_prod64(size_t size, ConstBufType<FpType> v0,
+                                           ConstBufType<FpType> v1) {
+        double re, im;
+        RP_CALL(rp_vec_prod(internal::rp_prec<FpType>(), v0.ptr, v1.ptr, size, &re, &im));
+        return {re, im};
+    }
+
+    template<RealScalar FpType>
+    static double vec_norm64(size_t size, ConstBufType<FpType> v) {
+        double n;
+        RP_CALL(rp_vec_norm(internal::rp_prec<FpType>(), v.ptr, size, &n));
+        return n;
+    }
+
+    template<RealScalar FpType>
+    static complex<FpType> vec_prod(size_t size, ConstBufType<FpType> v0,
+                                    ConstBufType<FpType> v1) {
+        auto c = vec_prod64<FpType>(size, v0, v1);
+        return complex<FpType>{(FpType)c.real(), (FpType)c.imag()};
+    }
+
+    template<RealScalar FpType>
+    static FpType vec_norm(size_t size, ConstBufType<FpType> v) {
+        return (FpType)vec_norm64<FpType>(size, v);
+    }
+
+    // res += ops * vec (GPUImpl::apply_ops, src/krnl.cu:220-257)
+    template<RealScalar FpType>
+    static void apply_ops(typename SpinOp<FpType>::IndexType len, BufType<FpType> res,
+                          const DevSumOps<FpType> &ops, ConstBufType<FpType> vec) {
+        RP_CALL(rp_apply_ops(ops.device_terms(len), res.ptr, vec.ptr, 1));
+    }
+
+    // res = ops * vec, with the zeroing fused into the apply
+    template<RealScalar FpType>
+    static void apply_ops_overwrite(typename SpinOp<FpType>::IndexType len, BufType<FpType> res,
+                                    const DevSumOps<FpType> &ops, ConstBufType<FpType> vec) {
+        RP_CALL(rp_apply_ops(ops.device_terms(len), res.ptr, vec.ptr, 0));
+    }
+
+    // one fused Gram-Schmidt pass of w against V[0..k): h = V^H w (FP64),
+    // w -= V h, returns ||w|| (FP64)
+    template<RealScalar FpType>
+    static double gs_pass(size_t size, const std::vector<const DeviceVec<FpType> *> &V,
+                          BufType<FpType> w, std::vector<std::complex<double>> &h) {
+        std::vector<const void *> p(V.size());
+        for (size_t i = 0; i < V.size(); i++) p[i] = V[i]->ptr;
+        h.resize(V.size());
+        double nrm;
+        RP_CALL(rp_gs_pass(internal::rp_prec<FpType>(), p.data(), (int)V.size(), w.ptr, size,
+                           (double *)h.data(), &nrm));
+        return nrm;
+    }
+
+    // y += sum_i c[i] V[i] in one sweep
+    template<RealScalar FpType>
+    static void lincomb(size_t size, BufType<FpType> y,
+                        const std::vector<const DeviceVec<FpType> *> &V,
+                        const std::vector<std::complex<double>> &c) {
+        std::vector<const void *> p(V.size());
+        for (size_t i = 0; i < V.size(); i++) p[i] = V[i]->ptr;
+        RP_CALL(rp_lincomb(internal::rp_prec<FpType>(), y.ptr, p.data(), (int)V.size(),
+                           (const double *)c.data(), size));
+    }
+};
+
+template<RealScalar FpType>
+inline rp_terms_t DevSumOps<FpType>::device_terms(int len) const {
+    if (handle && handle_len == len) return handle;
+    release();
+    GPUImpl::ensure_context();
+    std::vector<u64> bits;
+    std::vector<double> coeff;
+    this->export_terms(bits, coeff);
+    RP_CALL(rp_terms_upload(&handle, internal::rp_prec<FpType>(), len, bits.data(), coeff.data(),
+                            bits.size()));
+    handle_len = len;
+    return handle;
+}

@@ -1,0 +1,77 @@
+// reapers.h -- umbrella header of the REAPERS C++ API on the MI355X backend.
+//
+// Mirrors the reference's inc/reapers.h:22-123: namespace REAPERS, DefFpType
+// (float unless REAPERS_FP64), complex = std::complex, SumOps = the backend's
+// term list, State, EvolutionAlgorithm::Krylov, Model::{FermionOp, SYK}.  The
+// backend is GPUImpl over libreapers_hip.so's C ABI; link with
+// -lreapers_hip.  Host code only: no HIP headers are needed to use it.
+#pragma once
+
+#include <algorithm>
+#include <cassert>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <concepts>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <exception>
+#include <iostream>
+#include <memory>
+#include <random>
+#include <set>
+#include <sstream>
+#include <string>
+#include <tuple>
+#include <type_traits>
+#include <vector>
+
+#include "reapers_hip.h"
+
+#define _REAPERS_H_
+
+namespace REAPERS {
+
+template<typename T>
+concept RealScalar = std::floating_point<T>;
+
+#if defined(REAPERS_FP64) && defined(REAPERS_FP32)
+#error "You can only define one of REAPERS_FP64 or REAPERS_FP32"
+#elif defined(REAPERS_FP64)
+using DefFpType = double;
+#else
+using DefFpType = float;
+#endif
+
+template<RealScalar FpType = DefFpType>
+using complex = std::complex<FpType>;
+
+template<RealScalar FpType = DefFpType>
+constexpr FpType epsilon() { return std::numeric_limits<FpType>::epsilon(); }
+
+template<typename T>
+concept ComplexScalar = std::convertible_to<T, std::complex<double>>;
+
+template<typename T>
+concept ScalarType = RealScalar<T> || ComplexScalar<T>;
+
+#include "except.h"
+#include "ops.h"
+#include "sumops.h"
+#include "expm.h"
+#include "gpuimpl.h"
+
+using DefImpl = GPUImpl;
+
+template<RealScalar FpType = DefFpType>
+using SumOps = DefImpl::SumOps<FpType>;
+
+#include "state.h"
+
+namespace Model {
+#include "fermions.h"
+#include "syk.h"
+}
+
+}  // namespace REAPERS

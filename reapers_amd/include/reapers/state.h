@@ -1,0 +1,285 @@
+// state.h -- State: a vector in the 2^len Hilbert space of a spin chain,
+// owning a ring of backend buffers (the reference's State,
+// inc/state.h:43-443).  buf(i) is bufs[(curbuf + i) % nbufs]; evolution
+// algorithms enlarge the ring and rotate curbuf instead of copying vectors.
+#pragma once
+
+#ifndef _REAPERS_H_
+#error "Do not include this file directly. Include the master header file reapers.h"
+#endif
+
+template<RealScalar FpType, typename Impl>
+class State;
+
+// statistics and options of a restarted Krylov evolution
+struct KrylovStats {
+    long matvecs = 0;
+    long restarts = 0;
+    long gs_passes = 0;
+    int breakdown = 0;
+    int substeps = 0;
+    double last_nrm = 0;
+    double host_ms = 0;       // time in the host-side small-matrix work
+};
+
+enum class Reorth { Selective, Always };
+
+struct KrylovOptions {
+    int krydim = 5;
+    double tol = 0;           // 0: epsilon of the state's precision
+    int max_iters = 0;
+    Reorth reorth = Reorth::Selective;
+    double substep = 0;       // 0: whole t in one evolution
+};
+
+namespace EvolutionAlgorithm {
+#include "algo.h"
+}
+
+template<typename Impl>
+using DefEvolutionAlgorithm = EvolutionAlgorithm::Krylov<Impl>;
+
+template<RealScalar FpType = DefFpType, typename Impl = DefImpl>
+class State {
+    template<RealScalar F1, typename I1> friend class State;
+
+    using IndexType = size_t;
+    typename SpinOp<FpType>::IndexType len;
+    int curbuf;
+    std::vector<typename Impl::template VecType<FpType>> bufs;
+
+public:
+    State() : len(0), curbuf(0) {}
+
+    State(typename SpinOp<FpType>::IndexType len) : len(len), curbuf(0) {
+        if (len == 0) ThrowException(InvalidArgument, "len", "cannot be zero");
+        if (len > NUM_SPIN_SITES)
+            ThrowException(InvalidArgument, "len", "cannot be larger than NUM_SPIN_SITES");
+        bufs.emplace_back(dim());
+    }
+
+    State(const State &st) : len(st.len), curbuf(0) {
+        if (!len) return;
+        bufs.emplace_back(dim());
+        Impl::copy_vec(dim(), buf(), st.buf());
+    }
+
+    State(State &&st) : State() {
+        std::swap(len, st.len);
+        std::swap(bufs, st.bufs);
+        std::swap(curbuf, st.curbuf);
+    }
+
+    ~State() { len = 0; curbuf = 0; }
+
+    // precision conversion (inc/state.h:104-109); double -> float rounds to
+    // nearest unless another mode is given
+    template<RealScalar F1>
+    explicit State(const State<F1, Impl> &st, int rounding = RP_ROUND_NEAREST)
+        : len(st.len), curbuf(0) {
+        if (!len) return;
+        bufs.emplace_back(dim());
+        Impl::copy_vec(dim(), buf(), st.buf(), rounding);
+    }
+
+    void random_state(u64 seed = 0) {
+        if (!len) return;
+        Impl::init_random(dim(), buf(), seed);
+    }
+
+    void zero_state() {
+        if (!len) return;
+        Impl::zero_vec(dim(), buf());
+    }
+
+    int num_bufs() const { return (int)bufs.size(); }
+    auto spin_chain_length() const { return len; }
+    IndexType dim() const { return (IndexType)1 << len; }
+    size_t size() const { return len ? dim() : 0; }
+
+    typename Impl::template BufType<FpType> buf(int i = 0) {
+        assert(len != 0 && num_bufs() >= i + 1);
+        return bufs[(curbuf + i) % num_bufs()].get();
+    }
+    typename Impl::template ConstBufType<FpType> buf(int i = 0) const {
+        assert(len != 0 && num_bufs() >= i + 1);
+        return bufs[(curbuf + i) % num_bufs()].get();
+    }
+
+    void inc_curbuf(int i = 1) {
+        if (!len) return;
+        curbuf = (curbuf + i) % num_bufs();
+    }
+
+    void enlarge(int n) {
+        if (!len) return;
+        int nbufs = num_bufs();
+        for (int i = nbufs; i < n; i++) bufs.emplace_back(dim());
+    }
+
+    void swap_bufs(int i, int j) {
+        if (!len) return;
+        auto n = num_bufs();
+        i = ((i + curbuf) % n + n) % n;
+        j = ((j + curbuf) % n + n) % n;
+        if (i != j) swap(bufs[i], bufs[j]);
+    }
+
+    complex<FpType> operator[](IndexType i) const {
+        if (i >= dim()) ThrowException(StateIndexTooLarge, i, dim() - 1);
+        return buf()[i];
+    }
+
+    // whole-vector host transfers (interleaved complex<FpType>)
+    void load(const complex<FpType> *host) { buf().upload(host); }
+    void store(complex<FpType> *host) const { buf().download(host); }
+
+    State &operator=(const State &st) {
+        if (this == &st) return *this;
+        if (dim() != st.dim() || !len) {
+            bufs.clear();
+            curbuf = 0;
+            len = st.len;
+            if (len) bufs.emplace_back(dim());
+        }
+        if (len) Impl::copy_vec(dim(), buf(), st.buf());
+        return *this;
+    }
+
+    State &operator=(State &&st) {
+        if (this == &st) return *this;
+        std::swap(len, st.len);
+        std::swap(curbuf, st.curbuf);
+        std::swap(bufs, st.bufs);
+        return *this;
+    }
+
+    template<RealScalar F1>
+    State &operator=(const State<F1, Impl> &st) {
+        if (dim() != st.dim() || !len) {
+            bufs.clear();
+            curbuf = 0;
+            len = st.len;
+            if (len) bufs.emplace_back(dim());
+        }
+        if (len) Impl::copy_vec(dim(), buf(), st.buf());
+        return *this;
+    }
+
+    State &operator+=(const State &rhs) {
+        if (len != rhs.len) ThrowException(InvalidArgument, "dimensions of states must match");
+        if (len) Impl::add_vec(dim(), buf(), rhs.buf());
+        return *this;
+    }
+
+    State operator+(const State &rhs) const {
+        if (len != rhs.len) ThrowException(InvalidArgument, "dimensions of states must match");
+        if (!len) return {};
+        State res(rhs.len);
+        Impl::add_vec(dim(), res.buf(), buf(), rhs.buf());
+        return res;
+    }
+
+    State &operator-=(const State &rhs) {
+        if (len != rhs.len) ThrowException(InvalidArgument, "dimensions of states must match");
+        if (len) Impl::add_vec(dim(), buf(), FpType(-1.0), rhs.buf());
+        return *this;
+    }
+
+    State operator-(const State &rhs) const {
+        if (!len) return {};
+        State res{*this};
+        res -= rhs;
+        return res;
+    }
+
+    // s = ops * s, through a second ring buffer (inc/state.h:291-302)
+    State &operator*=(const typename Impl::template SumOps<FpType> &ops) {
+        if (!len) return *this;
+        enlarge(2);
+        auto &v = buf();
+        inc_curbuf();
+        Impl::apply_ops_overwrite(len, buf(), ops, v);
+        return *this;
+    }
+
+    State &operator*=(complex<FpType> s) {
+        if (len) Impl::scale_vec(dim(), buf(), s);
+        return *this;
+    }
+
+    FpType norm() const {
+        if (!len) return {};
+        return Impl::vec_norm(dim(), buf());
+    }
+
+    double norm64() const {
+        if (!len) return 0;
+        return Impl::vec_norm64(dim(), buf());
+    }
+
+    void normalize() {
+        if (!len) return;
+        Impl::scale_vec(dim(), buf(), (FpType)(1.0 / norm64()));
+    }
+
+    bool operator==(const State &st) const {
+        if (len != st.len) return false;
+        if (!len) return true;
+        State diff{*this - st};
+        return diff.norm() == 0.0;
+    }
+
+    friend State operator*(const typename Impl::template SumOps<FpType> &ops, const State &s) {
+        if (!s.len) return {};
+        State res(s.len);
+        Impl::apply_ops_overwrite(s.len, res.buf(), ops, s.buf());
+        return res;
+    }
+
+    // <s0|s1> (inc/state.h:368-375)
+    friend complex<FpType> operator*(const State &s0, const State &s1) {
+        if (s0.dim() != s1.dim()) ThrowException(InvalidArgument, "Dimensions of states", "must match");
+        if (!s0.len) return {};
+        return Impl::vec_prod(s0.dim(), s0.buf(), s1.buf());
+    }
+
+    std::complex<double> inner64(const State &s1) const {
+        if (dim() != s1.dim()) ThrowException(InvalidArgument, "Dimensions of states", "must match");
+        return Impl::vec_prod64(dim(), buf(), s1.buf());
+    }
+
+    template<template<typename> typename Algo = DefEvolutionAlgorithm, typename... Args>
+    FpType ground_state(const typename Impl::template SumOps<FpType> &ham, Args &&...args) {
+        if (!len) return {};
+        return Algo<Impl>::ground_state(*this, ham, std::forward<Args>(args)...);
+    }
+
+    // |psi> = exp(-(i t + beta) H) |psi> (inc/state.h:413-424)
+    template<template<typename> typename Algo = DefEvolutionAlgorithm, RealScalar F1,
+             RealScalar F2, typename... Args>
+    void evolve(const typename Impl::template SumOps<FpType> &ham, F1 t, F2 beta = 0.0,
+                Args &&...args) {
+        if ((t == 0.0) && (beta == 0.0)) return;
+        if (!len) return;
+        Algo<Impl>::evolve(*this, ham, static_cast<FpType>(t), static_cast<FpType>(beta),
+                           std::forward<Args>(args)...);
+    }
+
+    // evolve with options and statistics (FP64 times)
+    KrylovStats evolve_ex(const typename Impl::template SumOps<FpType> &ham, double t,
+                          double beta, const KrylovOptions &opt) {
+        KrylovStats st;
+        if ((t == 0.0) && (beta == 0.0)) return st;
+        if (!len) return st;
+        EvolutionAlgorithm::Krylov<Impl>::evolve_ex(*this, ham, t, beta, opt, st);
+        return st;
+    }
+
+    void gc() {
+        if (!len) return;
+        if (curbuf != 0) swap_bufs(0, -curbuf);
+        while (num_bufs() > 1) bufs.pop_back();
+        curbuf = 0;
+    }
+};
