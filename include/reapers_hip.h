@@ -115,6 +115,12 @@ typedef struct rp_terms *rp_terms_t;
  * uses.  len is the spin chain length (1..32). */
 int rp_terms_upload(rp_terms_t *out, int prec, int len, const uint64_t *bits,
                     const double *coeff, uint64_t nterms);
+/* rp_terms_upload with options: RP_TERMS_ACC_NATIVE accumulates a complex<float>
+ * apply in FP32 instead of FP64 (faster, one FP32 rounding per term instead
+ * of one per output element). */
+#define RP_TERMS_ACC_NATIVE 1
+int rp_terms_upload_ex(rp_terms_t *out, int prec, int len, const uint64_t *bits,
+                       const double *coeff, uint64_t nterms, int flags);
 /* DevSumOps::release (inc/gpuimpl.h:1087-1105) */
 int rp_terms_free(rp_terms_t h);
 

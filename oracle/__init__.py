@@ -51,6 +51,9 @@ def lib():
                                                _c.c_int64]
         for n in ("or_apply_f64", "or_apply_f32"):
             getattr(L, n).argtypes = [_c.c_int, _vp, _vp, _c.c_int64, _vp, _vp]
+        for n in ("or_apply_rows_f64", "or_apply_rows_f32"):
+            getattr(L, n).argtypes = [_c.c_int, _vp, _vp, _c.c_int64, _vp, _vp, _c.c_int64,
+                                      _c.c_int64]
         L.or_dense_matrix_f64.argtypes = [_c.c_int, _vp, _vp, _c.c_int64, _vp]
         L.or_expm_f64.argtypes = [_c.c_int, _vp, _vp]
         L.or_expm_f32.argtypes = [_c.c_int, _vp, _vp]
@@ -165,6 +168,22 @@ def apply(length, bits, coeff, psi, fp=64, out=None):
     f = lib().or_apply_f32 if fp == 32 else lib().or_apply_f64
     f(length, _p(bits), _p(c), len(bits), _p(vin), _p(vout))
     return _join(vout)
+
+
+def apply_rows_timed(length, bits, coeff, psi, row_end, fp=32):
+    """Time rows [0, row_end) of out += H psi (the reference's CPU apply) and
+    return (seconds, out rows).  psi must already have the fp's dtype layout
+    (complex64 for fp=32)."""
+    import time
+    bits = np.ascontiguousarray(bits, np.uint64)
+    dt = np.float32 if fp == 32 else np.float64
+    c = _split(coeff, dt)
+    vin = np.ascontiguousarray(psi).view(dt)
+    vout = np.zeros(2 * row_end, dt)
+    f = lib().or_apply_rows_f32 if fp == 32 else lib().or_apply_rows_f64
+    t0 = time.perf_counter()
+    f(length, _p(bits), _p(c), len(bits), _p(vin), _p(vout), 0, row_end)
+    return time.perf_counter() - t0, _join(vout)
 
 
 def dense_matrix(length, bits, coeff):

@@ -88,6 +88,11 @@ void or_apply_f64(int len, const uint64_t *bits, const double *coeff, int64_t nt
                   const double *in, double *out);
 void or_apply_f32(int len, const uint64_t *bits, const float *coeff, int64_t nterms,
                   const float *in, float *out);
+/* rows [row_begin, row_end) only: a bounded sample of one apply, for timing */
+void or_apply_rows_f64(int len, const uint64_t *bits, const double *coeff, int64_t nterms,
+                       const double *in, double *out, int64_t row_begin, int64_t row_end);
+void or_apply_rows_f32(int len, const uint64_t *bits, const float *coeff, int64_t nterms,
+                       const float *in, float *out, int64_t row_begin, int64_t row_end);
 /* dense matrix, column-major dim x dim complex (inc/cpuimpl.h:372-390) */
 void or_dense_matrix_f64(int len, const uint64_t *bits, const double *coeff,
                          int64_t nterms, double *mat);

@@ -12,17 +12,26 @@
 //   z = phase mask (Z bits, odd positions),
 //   c = coeff * (-1)^popcount(x & z), so that
 //     res[m] += c * (-1)^popcount(m & z) * vec[m ^ x],
-// and sorts the terms by x: terms with equal x share one gather of vec, and
-// terms whose x agree above the tile bits b share one input tile.
+// and sorts the terms by x.  Terms with equal x ("subs") share one gather of
+// vec; subs whose x agree above the tile bits b ("groups") share one staged
+// input tile.  A Hermitian term has c purely real or purely imaginary, so
+// single-term subs are classified real / imaginary / complex on the host and
+// the kernel spends 2 FMAs (not 4) per term and amplitude on the first two.
 //
-// Tiled kernel: one workgroup owns the output tile [T, T + 2^b).  For each
-// distinct high part xh = x >> b it stages the input tile T ^ (xh << b) in
-// LDS with coalesced loads (an XOR with xh maps an aligned tile onto an
-// aligned tile), then applies every term of that group from LDS: the low
-// bits of x permute within the tile.  The output accumulates in registers and
-// is written once, with the caller's zero_vec fused in when accumulate == 0.
-// HBM traffic per apply: (G_b + 1) * 2^len * S bytes, G_b = number of
-// distinct xh.
+// Tiled kernel: one workgroup of NT threads owns the output tile
+// [t0, t0 + 2^b); thread tid holds amplitudes t0 + r*NT + tid, r < RP.  For
+// each group it stages the input tile t0 ^ (xh << b) in LDS with coalesced
+// loads (an XOR with the high part maps an aligned tile onto an aligned
+// tile), then applies every sub of the group from LDS: the low bits of x
+// permute lanes (tid ^ (x mod NT)) and rows (r ^ (x / NT)).  The sign
+// (-1)^popcount(m & z) splits into a per-workgroup part (t0 & z), a per-lane
+// part (tid & z) computed once per sub, and a per-row part (r*NT & z) that is
+// uniform across the wavefront.  The output accumulates in registers (FP64
+// for both state precisions by default) and is written once, with the
+// caller's zero_vec fused in when accumulate == 0.
+// Workgroups are numbered so that each XCD walks a contiguous range of
+// output tiles, whose input tiles overlap in that XCD's L2.
+// HBM traffic per apply: (G_b + 1) * 2^len * S bytes, G_b = number of groups.
 #include "rp_internal.h"
 
 #include <algorithm>
@@ -35,12 +44,17 @@ template <typename R> struct CxA;
 template <> struct CxA<float> { using T = float2; };
 template <> struct CxA<double> { using T = double2; };
 
-struct GroupRec { uint32_t xh, sub_begin, sub_end, pad; };
-struct SubRec { uint32_t xl, term_begin, term_end, pad; };
-template <typename R> struct TermRec { uint32_t z, pad; R cr, ci; };
-template <typename R> struct FlatTerm { uint32_t x, z; R cr, ci; };
+// kinds of subs: one term with real / imaginary / complex c, or several terms
+enum : uint32_t { kSubReal = 0, kSubImag = 1, kSubComplex = 2, kSubMulti = 3 };
 
-constexpr int kApplyThreads = 256;
+template <typename A> struct SubRec {
+    uint32_t xl, z, kind, term_begin, term_end, pad0, pad1, pad2;
+    A cr, ci;
+};
+template <typename A> struct TermRec { uint32_t z, pad; A cr, ci; };
+template <typename A> struct FlatTerm { uint32_t x, z; A cr, ci; };
+
+constexpr int kSmallThreads = 256;
 
 }  // namespace rp
 
@@ -48,103 +62,252 @@ struct rp_terms {
     int prec = RP_C64;
     int len = 0;
     int tile_bits = 0;       // 0: small kernel (one thread per amplitude)
+    int threads = 0;         // workgroup size of the tiled kernel
+    int acc64 = 1;           // accumulate in FP64
     uint64_t nterms = 0, nflips = 0, ngroups = 0, nsubs = 0;
     void *d_groups = nullptr, *d_subs = nullptr, *d_terms = nullptr, *d_flat = nullptr;
 };
 
 namespace rp {
 
-__device__ inline int parity32(uint32_t v) { return __popc(v) & 1; }
+__device__ inline uint32_t parity32(uint32_t v) { return __popc(v) & 1u; }
 
 // one thread per amplitude; terms in (x-sorted) flat order
-template <typename R>
-__global__ void __launch_bounds__(kApplyThreads)
+template <typename R, typename A>
+__global__ void __launch_bounds__(kSmallThreads)
 k_apply_small(const typename CxA<R>::T *__restrict__ in, typename CxA<R>::T *__restrict__ out,
-              uint32_t dim, const FlatTerm<R> *__restrict__ terms, int nterms, int accumulate)
+              uint32_t dim, const FlatTerm<A> *__restrict__ terms, int nterms, int accumulate)
 {
-    uint32_t m = blockIdx.x * kApplyThreads + threadIdx.x;
+    uint32_t m = blockIdx.x * kSmallThreads + threadIdx.x;
     if (m >= dim) return;
-    R ar = 0, ai = 0;
+    A ar = 0, ai = 0;
     for (int t = 0; t < nterms; t++) {
-        FlatTerm<R> f = terms[t];
-        auto v = in[m ^ f.x];
-        R cr = parity32(m & f.z) ? -f.cr : f.cr;
-        R ci = parity32(m & f.z) ? -f.ci : f.ci;
-        ar += cr * v.x - ci * v.y;
-        ai += cr * v.y + ci * v.x;
+        const FlatTerm<A> f = terms[t];
+        const auto v = in[m ^ f.x];
+        const A vr = v.x, vi = v.y;
+        const bool neg = parity32(m & f.z);
+        const A cr = neg ? -f.cr : f.cr;
+        const A ci = neg ? -f.ci : f.ci;
+        ar += cr * vr - ci * vi;
+        ai += cr * vi + ci * vr;
     }
     if (accumulate) {
-        auto o = out[m];
-        ar += o.x;
-        ai += o.y;
+        const auto o = out[m];
+        ar += (A)o.x;
+        ai += (A)o.y;
     }
     typename CxA<R>::T o;
-    o.x = ar;
-    o.y = ai;
+    o.x = (R)ar;
+    o.y = (R)ai;
     out[m] = o;
 }
 
-template <typename R, int TB>
-__global__ void __launch_bounds__(kApplyThreads)
-k_apply_tiled(const typename CxA<R>::T *__restrict__ in, typename CxA<R>::T *__restrict__ out,
-              const GroupRec *__restrict__ groups, int ngroups, const SubRec *__restrict__ subs,
-              const TermRec<R> *__restrict__ terms, int accumulate)
+// Sub classes, in the order a group stores its subs: single-term real c with
+// no phase bit in the row bits, real with row bits, imaginary (same split),
+// complex, and several terms per flip mask.
+enum : uint32_t { kRunReal0 = 0, kRunReal1, kRunImag0, kRunImag1, kRunCplx, kRunMulti, kRuns };
+
+struct GroupRuns { uint32_t xh, b[kRuns + 1]; };
+
+template <typename R, int TB, int NT>
+constexpr int apply_waves_per_eu()
+{
+    constexpr int lds = 2 * (1 << TB) * 2 * (int)sizeof(R);
+    constexpr int wg_by_lds = (160 * 1024) / lds;
+    constexpr int wg_by_waves = 1024 / NT;
+    constexpr int wg = wg_by_lds < wg_by_waves ? wg_by_lds : wg_by_waves;
+    constexpr int w = (wg < 1 ? 1 : wg) * (NT / 64) / 4;
+    return w < 1 ? 1 : w;
+}
+
+// Copy the input tile starting at element `src` into LDS buffer `dst` with
+// gfx950's asynchronous global->LDS loads: every lane moves 16 bytes, a
+// wavefront fills a contiguous 1 KiB run starting at the wave-uniform LDS
+// address.  Completion is tracked by the vector memory counter.
+template <typename T, int TB, int NT>
+__device__ inline void tile_dma(const T *__restrict__ in, uint64_t src, T *dst)
+{
+    constexpr int TILE_BYTES = (1 << TB) * (int)sizeof(T);
+    constexpr int PER_THREAD = TILE_BYTES / (16 * NT);
+    static_assert(PER_THREAD >= 1 && TILE_BYTES % (16 * NT) == 0, "tile too small for DMA");
+    const char *g = (const char *)(in + src);
+    char *l = (char *)dst;
+    const uint32_t tid = threadIdx.x, wave_base = tid & ~63u;
+#pragma unroll
+    for (int c = 0; c < PER_THREAD; c++) {
+        const uint32_t chunk = c * NT;
+        __builtin_amdgcn_global_load_lds(
+            (const void *)(g + (size_t)(chunk + tid) * 16),
+            (__attribute__((address_space(3))) void *)(l + (size_t)(chunk + wave_base) * 16),
+            16, 0, 0);
+    }
+}
+
+// Apply every sub of group G from the staged input tile `buf` (LDS) to the
+// RP accumulated rows of this thread.
+template <typename R, typename A, int TB, int NT>
+__device__ __forceinline__ void apply_group(const GroupRuns &G, const typename CxA<R>::T *buf,
+                                            const SubRec<A> *__restrict__ subs,
+                                            const TermRec<A> *__restrict__ terms, uint32_t t0,
+                                            uint32_t tid, A (&accr)[(1 << TB) / NT],
+                                            A (&acci)[(1 << TB) / NT])
 {
     using T = typename CxA<R>::T;
-    constexpr int NT = kApplyThreads;
-    constexpr int TILE = 1 << TB;
-    constexpr int RP = TILE / NT;          // amplitudes per thread
-    __shared__ T tile[TILE];
+    constexpr int RP = (1 << TB) / NT;
+    constexpr int LB = NT == 1024 ? 10 : NT == 512 ? 9 : NT == 256 ? 8 : 7;
 
-    const uint32_t t0 = (uint32_t)blockIdx.x << TB;      // output tile base
-    const uint32_t tid = threadIdx.x;
-    R accr[RP], acci[RP];
+#define RP_SUB_PROLOGUE                                                            \
+    const SubRec<A> S = subs[s];                                               \
+    const T *row = buf + (tid ^ (S.xl & (NT - 1)));                            \
+    const uint32_t xr = S.xl >> LB;                                            \
+    const bool lneg = parity32((t0 | tid) & S.z);                              \
+    const uint32_t zr = (S.z >> LB) & (RP - 1);                                \
+    (void)zr;
+    // real c, no phase bit among the row bits
+    for (uint32_t s = G.b[kRunReal0]; s < G.b[kRunReal0 + 1]; s++) {
+        RP_SUB_PROLOGUE
+        const A c = lneg ? -S.cr : S.cr;
 #pragma unroll
-    for (int r = 0; r < RP; r++) { accr[r] = 0; acci[r] = 0; }
-
-    for (int g = 0; g < ngroups; g++) {
-        const GroupRec G = groups[g];
-        const uint32_t src = t0 ^ (G.xh << TB);
-        __syncthreads();
+        for (int r = 0; r < RP; r++) {
+            const T v = row[(r ^ xr) * NT];
+            accr[r] += c * (A)v.x;
+            acci[r] += c * (A)v.y;
+        }
+    }
+    for (uint32_t s = G.b[kRunReal1]; s < G.b[kRunReal1 + 1]; s++) {
+        RP_SUB_PROLOGUE
+        const A c = lneg ? -S.cr : S.cr;
 #pragma unroll
-        for (int r = 0; r < RP; r++) tile[r * NT + tid] = in[(uint64_t)src + r * NT + tid];
-        __syncthreads();
-        for (uint32_t s = G.sub_begin; s < G.sub_end; s++) {
-            const SubRec S = subs[s];
-            const uint32_t lane_src = tid ^ (S.xl & (NT - 1));
-            const uint32_t xr = S.xl / NT;
-            R dr[RP], di[RP];
+        for (int r = 0; r < RP; r++) {
+            const T v = row[(r ^ xr) * NT];
+            const A cc = parity32(r & zr) ? -c : c;
+            accr[r] += cc * (A)v.x;
+            acci[r] += cc * (A)v.y;
+        }
+    }
+    // imaginary c = i ci: (i ci)(vr + i vi) = -ci vi + i ci vr
+    for (uint32_t s = G.b[kRunImag0]; s < G.b[kRunImag0 + 1]; s++) {
+        RP_SUB_PROLOGUE
+        const A c = lneg ? -S.ci : S.ci;
 #pragma unroll
-            for (int r = 0; r < RP; r++) { dr[r] = 0; di[r] = 0; }
+        for (int r = 0; r < RP; r++) {
+            const T v = row[(r ^ xr) * NT];
+            accr[r] -= c * (A)v.y;
+            acci[r] += c * (A)v.x;
+        }
+    }
+    for (uint32_t s = G.b[kRunImag1]; s < G.b[kRunImag1 + 1]; s++) {
+        RP_SUB_PROLOGUE
+        const A c = lneg ? -S.ci : S.ci;
+#pragma unroll
+        for (int r = 0; r < RP; r++) {
+            const T v = row[(r ^ xr) * NT];
+            const A cc = parity32(r & zr) ? -c : c;
+            accr[r] -= cc * (A)v.y;
+            acci[r] += cc * (A)v.x;
+        }
+    }
+    for (uint32_t s = G.b[kRunCplx]; s < G.b[kRunCplx + 1]; s++) {
+        RP_SUB_PROLOGUE
+        const A cr = lneg ? -S.cr : S.cr;
+        const A ci = lneg ? -S.ci : S.ci;
+#pragma unroll
+        for (int r = 0; r < RP; r++) {
+            const T v = row[(r ^ xr) * NT];
+            const bool n = parity32(r & zr);
+            const A a = n ? -cr : cr, c = n ? -ci : ci;
+            accr[r] += a * (A)v.x - c * (A)v.y;
+            acci[r] += a * (A)v.y + c * (A)v.x;
+        }
+    }
+    // several terms share x: d(m) = sum_t c_t (-1)^popcount(m & z_t),
+    // formed for four rows at a time to bound register use
+    for (uint32_t s = G.b[kRunMulti]; s < G.b[kRunMulti + 1]; s++) {
+        const SubRec<A> S = subs[s];
+        const T *row = buf + (tid ^ (S.xl & (NT - 1)));
+        const uint32_t xr = S.xl >> LB;
+        constexpr int RC = RP < 4 ? RP : 4;
+#pragma unroll
+        for (int r0 = 0; r0 < RP; r0 += RC) {
+            A dr[RC], di[RC];
+#pragma unroll
+            for (int q = 0; q < RC; q++) { dr[q] = 0; di[q] = 0; }
             for (uint32_t t = S.term_begin; t < S.term_end; t++) {
-                const TermRec<R> C = terms[t];
-                const uint32_t hp = parity32(t0 & C.z);
+                const TermRec<A> C = terms[t];
+                const bool lneg = parity32((t0 | tid) & C.z);
+                const A cr = lneg ? -C.cr : C.cr;
+                const A ci = lneg ? -C.ci : C.ci;
+                const uint32_t zr = (C.z >> LB) & (RP - 1);
 #pragma unroll
-                for (int r = 0; r < RP; r++) {
-                    const uint32_t p = parity32(((uint32_t)r * NT + tid) & C.z) ^ hp;
-                    dr[r] += p ? -C.cr : C.cr;
-                    di[r] += p ? -C.ci : C.ci;
+                for (int q = 0; q < RC; q++) {
+                    const bool n = parity32((r0 + q) & zr);
+                    dr[q] += n ? -cr : cr;
+                    di[q] += n ? -ci : ci;
                 }
             }
 #pragma unroll
-            for (int r = 0; r < RP; r++) {
-                const T v = tile[(r ^ xr) * NT + lane_src];
-                accr[r] += dr[r] * v.x - di[r] * v.y;
-                acci[r] += dr[r] * v.y + di[r] * v.x;
+            for (int q = 0; q < RC; q++) {
+                const T v = row[((r0 + q) ^ xr) * NT];
+                accr[r0 + q] += dr[q] * (A)v.x - di[q] * (A)v.y;
+                acci[r0 + q] += dr[q] * (A)v.y + di[q] * (A)v.x;
             }
         }
+    }
+#undef RP_SUB_PROLOGUE
+}
+
+template <typename R, typename A, int TB, int NT>
+__global__ void __launch_bounds__(NT, (apply_waves_per_eu<R, TB, NT>()))
+k_apply_tiled(const typename CxA<R>::T *__restrict__ in, typename CxA<R>::T *__restrict__ out,
+              const GroupRuns *__restrict__ groups, int ngroups, const SubRec<A> *__restrict__ subs,
+              const TermRec<A> *__restrict__ terms, int accumulate)
+{
+    using T = typename CxA<R>::T;
+    constexpr int TILE = 1 << TB;
+    constexpr int RP = TILE / NT;          // amplitudes per thread
+    constexpr int LB = NT == 1024 ? 10 : NT == 512 ? 9 : NT == 256 ? 8 : 7;
+    static_assert((1 << LB) == NT, "NT must be a power of two in [128, 1024]");
+    // two separate LDS tiles: while one is applied, the next group's input
+    // tile streams into the other
+    __shared__ T tile0[TILE];
+    __shared__ T tile1[TILE];
+
+    // XCD-aware numbering: consecutive workgroup ids go to different XCDs,
+    // so give each XCD a contiguous range of tiles
+    const uint32_t nb = gridDim.x, b = blockIdx.x;
+    const uint32_t tid_tile = (nb & 7u) == 0 ? (b & 7u) * (nb >> 3) + (b >> 3) : b;
+    const uint32_t t0 = tid_tile << TB;
+    const uint32_t tid = threadIdx.x;
+
+    A accr[RP], acci[RP];
+#pragma unroll
+    for (int r = 0; r < RP; r++) { accr[r] = 0; acci[r] = 0; }
+
+    if (ngroups > 0) tile_dma<T, TB, NT>(in, t0 ^ (groups[0].xh << TB), tile0);
+    for (int g = 0; g < ngroups; g += 2) {
+        // this thread's share of tile g has landed; the barrier makes every
+        // share visible and guarantees nobody still reads the other tile
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (g + 1 < ngroups) tile_dma<T, TB, NT>(in, t0 ^ (groups[g + 1].xh << TB), tile1);
+        apply_group<R, A, TB, NT>(groups[g], tile0, subs, terms, t0, tid, accr, acci);
+        if (g + 1 >= ngroups) break;
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (g + 2 < ngroups) tile_dma<T, TB, NT>(in, t0 ^ (groups[g + 2].xh << TB), tile0);
+        apply_group<R, A, TB, NT>(groups[g + 1], tile1, subs, terms, t0, tid, accr, acci);
     }
 #pragma unroll
     for (int r = 0; r < RP; r++) {
         const uint64_t m = (uint64_t)t0 + r * NT + tid;
         T o;
         if (accumulate) {
-            T p = out[m];
-            o.x = p.x + accr[r];
-            o.y = p.y + acci[r];
+            const T p = out[m];
+            o.x = (R)(accr[r] + (A)p.x);
+            o.y = (R)(acci[r] + (A)p.y);
         } else {
-            o.x = accr[r];
-            o.y = acci[r];
+            o.x = (R)accr[r];
+            o.y = (R)acci[r];
         }
         out[m] = o;
     }
@@ -164,23 +327,29 @@ static inline uint32_t pext_odd(uint64_t b)    // bit 2s+1 -> bit s
     return r;
 }
 
-static int choose_tile_bits(int len)
+// tile bits b and workgroup size: 8 amplitudes per thread, two 64 KiB LDS
+// tiles (2^13 complex<float> or 2^12 complex<double>) at full size, smaller
+// tiles and workgroups for short chains so that there are enough workgroups
+static void choose_tiling(int prec, int len, int *tb, int *nt)
 {
-    if (len >= 18) return 12;
-    if (len >= 14) return 10;
-    if (len >= 8) return 8;
-    return 0;
+    if (prec == RP_C64 && len >= 20) { *tb = 13; *nt = 1024; return; }
+    if (len >= 18) { *tb = 12; *nt = 512; return; }
+    if (len >= 14) { *tb = 10; *nt = 128; return; }
+    if (len >= 10) { *tb = 8; *nt = 128; return; }
+    *tb = 0;
+    *nt = kSmallThreads;
 }
 
-template <typename R>
+template <typename R, typename A>
 static int upload_impl(rp_terms *h, const uint64_t *bits, const double *coeff, uint64_t n)
 {
-    struct Tmp { uint32_t x, z; R cr, ci; uint64_t order; };
+    struct Tmp { uint32_t x, z; A cr, ci; };
     std::vector<Tmp> v;
     v.reserve(n);
     const uint64_t sitemask = h->len == 32 ? ~0ull : ((1ull << (2 * h->len)) - 1);
     for (uint64_t t = 0; t < n; t++) {
-        R cr = (R)coeff[2 * t], ci = (R)coeff[2 * t + 1];
+        // coefficients arrive as doubles holding values of the SumOps' FpType
+        A cr = (A)(R)coeff[2 * t], ci = (A)(R)coeff[2 * t + 1];
         if (cr == 0 && ci == 0) continue;
         if (bits[t] & ~sitemask)
             return set_error(RP_ERR_INVALID_ARGUMENT,
@@ -192,7 +361,6 @@ static int upload_impl(rp_terms *h, const uint64_t *bits, const double *coeff, u
         if (__builtin_popcount(e.x & e.z) & 1) { cr = -cr; ci = -ci; }
         e.cr = cr;
         e.ci = ci;
-        e.order = t;
         v.push_back(e);
     }
     const int tb = h->tile_bits;
@@ -203,26 +371,57 @@ static int upload_impl(rp_terms *h, const uint64_t *bits, const double *coeff, u
         return (a.x & lowmask) < (b.x & lowmask);
     });
     h->nterms = v.size();
-    std::vector<GroupRec> groups;
-    std::vector<SubRec> subs;
-    std::vector<TermRec<R>> terms;
-    std::vector<FlatTerm<R>> flat;
+    std::vector<GroupRuns> groups;
+    std::vector<SubRec<A>> subs;
+    std::vector<TermRec<A>> terms;
+    std::vector<FlatTerm<A>> flat;
     uint64_t nflips = 0;
     for (size_t i = 0; i < v.size(); i++) {
         if (i == 0 || v[i].x != v[i - 1].x) nflips++;
         flat.push_back({v[i].x, v[i].z, v[i].cr, v[i].ci});
-        if (!tb) continue;
-        uint32_t xh = v[i].x >> tb, xl = v[i].x & lowmask;
-        if (groups.empty() || groups.back().xh != xh) {
-            groups.push_back({xh, (uint32_t)subs.size(), (uint32_t)subs.size(), 0});
+    }
+    if (tb) {
+        const int lb = __builtin_ctz((unsigned)h->threads);
+        const uint32_t rpmask = (1u << (tb - lb)) - 1;
+        size_t i = 0;
+        while (i < v.size()) {
+            const uint32_t xh = v[i].x >> tb;
+            // the group's subs, classified, then stored run by run
+            std::vector<SubRec<A>> run[kRuns];
+            while (i < v.size() && (v[i].x >> tb) == xh) {
+                size_t j = i;
+                while (j < v.size() && v[j].x == v[i].x) j++;
+                SubRec<A> S;
+                memset(&S, 0, sizeof(S));
+                S.xl = v[i].x & lowmask;
+                int cls;
+                if (j - i == 1) {
+                    S.z = v[i].z;
+                    S.cr = v[i].cr;
+                    S.ci = v[i].ci;
+                    const bool rowz = ((S.z >> lb) & rpmask) != 0;
+                    if (v[i].ci == 0) { S.kind = kSubReal; cls = rowz ? kRunReal1 : kRunReal0; }
+                    else if (v[i].cr == 0) { S.kind = kSubImag; cls = rowz ? kRunImag1 : kRunImag0; }
+                    else { S.kind = kSubComplex; cls = kRunCplx; }
+                } else {
+                    S.kind = kSubMulti;
+                    S.term_begin = (uint32_t)terms.size();
+                    for (size_t q = i; q < j; q++) terms.push_back({v[q].z, 0, v[q].cr, v[q].ci});
+                    S.term_end = (uint32_t)terms.size();
+                    cls = kRunMulti;
+                }
+                run[cls].push_back(S);
+                i = j;
+            }
+            GroupRuns G;
+            G.xh = xh;
+            for (int k = 0; k < kRuns; k++) {
+                G.b[k] = (uint32_t)subs.size();
+                subs.insert(subs.end(), run[k].begin(), run[k].end());
+            }
+            G.b[kRuns] = (uint32_t)subs.size();
+            groups.push_back(G);
         }
-        if (subs.size() == groups.back().sub_begin || groups.back().sub_end == groups.back().sub_begin ||
-            subs.back().xl != xl) {
-            subs.push_back({xl, (uint32_t)terms.size(), (uint32_t)terms.size(), 0});
-            groups.back().sub_end = (uint32_t)subs.size();
-        }
-        terms.push_back({v[i].z, 0, v[i].cr, v[i].ci});
-        subs.back().term_end = (uint32_t)terms.size();
     }
     h->nflips = nflips;
     h->ngroups = tb ? groups.size() : 1;
@@ -240,11 +439,11 @@ static int upload_impl(rp_terms *h, const uint64_t *bits, const double *coeff, u
         return RP_OK;
     };
     int st;
-    if ((st = up(&h->d_flat, flat.data(), flat.size() * sizeof(FlatTerm<R>)))) return st;
+    if ((st = up(&h->d_flat, flat.data(), flat.size() * sizeof(FlatTerm<A>)))) return st;
     if (tb) {
-        if ((st = up(&h->d_groups, groups.data(), groups.size() * sizeof(GroupRec)))) return st;
-        if ((st = up(&h->d_subs, subs.data(), subs.size() * sizeof(SubRec)))) return st;
-        if ((st = up(&h->d_terms, terms.data(), terms.size() * sizeof(TermRec<R>)))) return st;
+        if ((st = up(&h->d_groups, groups.data(), groups.size() * sizeof(GroupRuns)))) return st;
+        if ((st = up(&h->d_subs, subs.data(), subs.size() * sizeof(SubRec<A>)))) return st;
+        if ((st = up(&h->d_terms, terms.data(), terms.size() * sizeof(TermRec<A>)))) return st;
     }
     RP_CHECK_HIP(hipStreamSynchronize(c.stream));
     return RP_OK;
@@ -260,45 +459,44 @@ static void free_handle(rp_terms *h)
     delete h;
 }
 
-template <typename R>
+template <typename R, typename A, int TB, int NT>
+static int launch_tiled(rp_terms *h, void *res, const void *vec, int accumulate)
+{
+    using T = typename CxA<R>::T;
+    const uint64_t grid = (1ull << h->len) >> TB;
+    RP_LAUNCH(RP_KERNEL_APPLY, (k_apply_tiled<R, A, TB, NT>), dim3((unsigned)grid), dim3(NT), 0,
+              (const T *)vec, (T *)res, (const GroupRuns *)h->d_groups, (int)h->ngroups,
+              (const SubRec<A> *)h->d_subs, (const TermRec<A> *)h->d_terms, accumulate);
+    return RP_OK;
+}
+
+template <typename R, typename A>
 static int apply_impl(rp_terms *h, void *res, const void *vec, int accumulate)
 {
     using T = typename CxA<R>::T;
     const uint64_t dim = 1ull << h->len;
     if (h->nterms == 0) {
-        if (!accumulate)
-            RP_CHECK_HIP(hipMemsetAsync(res, 0, dim * sizeof(T), ctx().stream));
+        if (!accumulate) RP_CHECK_HIP(hipMemsetAsync(res, 0, dim * sizeof(T), ctx().stream));
         return RP_OK;
     }
     if (h->tile_bits == 0) {
-        int grid = (int)((dim + kApplyThreads - 1) / kApplyThreads);
-        RP_LAUNCH(RP_KERNEL_APPLY, k_apply_small<R>, dim3(grid), dim3(kApplyThreads), 0,
-                  (const T *)vec, (T *)res, (uint32_t)dim, (const FlatTerm<R> *)h->d_flat,
+        int grid = (int)((dim + kSmallThreads - 1) / kSmallThreads);
+        RP_LAUNCH(RP_KERNEL_APPLY, (k_apply_small<R, A>), dim3(grid), dim3(kSmallThreads), 0,
+                  (const T *)vec, (T *)res, (uint32_t)dim, (const FlatTerm<A> *)h->d_flat,
                   (int)h->nterms, accumulate);
         return RP_OK;
     }
-    const uint64_t grid = dim >> h->tile_bits;
-    const GroupRec *g = (const GroupRec *)h->d_groups;
-    const SubRec *s = (const SubRec *)h->d_subs;
-    const TermRec<R> *t = (const TermRec<R> *)h->d_terms;
-    const int ng = (int)h->ngroups;
-    switch (h->tile_bits) {
-    case 8:
-        RP_LAUNCH(RP_KERNEL_APPLY, (k_apply_tiled<R, 8>), dim3((unsigned)grid), dim3(kApplyThreads),
-                  0, (const T *)vec, (T *)res, g, ng, s, t, accumulate);
+    switch (h->tile_bits * 10000 + h->threads) {
+    case 80128: return launch_tiled<R, A, 8, 128>(h, res, vec, accumulate);
+    case 100128: return launch_tiled<R, A, 10, 128>(h, res, vec, accumulate);
+    case 120512: return launch_tiled<R, A, 12, 512>(h, res, vec, accumulate);
+    case 131024:
+        if constexpr (sizeof(R) == 4) return launch_tiled<R, A, 13, 1024>(h, res, vec, accumulate);
         break;
-    case 10:
-        RP_LAUNCH(RP_KERNEL_APPLY, (k_apply_tiled<R, 10>), dim3((unsigned)grid), dim3(kApplyThreads),
-                  0, (const T *)vec, (T *)res, g, ng, s, t, accumulate);
-        break;
-    case 12:
-        RP_LAUNCH(RP_KERNEL_APPLY, (k_apply_tiled<R, 12>), dim3((unsigned)grid), dim3(kApplyThreads),
-                  0, (const T *)vec, (T *)res, g, ng, s, t, accumulate);
-        break;
-    default:
-        return set_error(RP_ERR_RUNTIME, "rp_apply_ops: unsupported tile size %d", h->tile_bits);
+    default: break;
     }
-    return RP_OK;
+    return set_error(RP_ERR_RUNTIME, "rp_apply_ops: unsupported tiling b=%d threads=%d",
+                     h->tile_bits, h->threads);
 }
 
 }  // namespace rp
@@ -307,8 +505,8 @@ using namespace rp;
 
 extern "C" {
 
-int rp_terms_upload(rp_terms_t *out, int prec, int len, const uint64_t *bits,
-                    const double *coeff, uint64_t nterms)
+int rp_terms_upload_ex(rp_terms_t *out, int prec, int len, const uint64_t *bits,
+                       const double *coeff, uint64_t nterms, int flags)
 {
     int st = check_init();
     if (st) return st;
@@ -317,18 +515,27 @@ int rp_terms_upload(rp_terms_t *out, int prec, int len, const uint64_t *bits,
     RP_REQUIRE(valid_prec(prec), "bad precision");
     RP_REQUIRE(len >= 1 && len <= 32, "len must be in [1, 32]");
     RP_REQUIRE(nterms == 0 || (bits && coeff), "null term arrays");
+    RP_REQUIRE((flags & ~RP_TERMS_ACC_NATIVE) == 0, "unknown flags");
     rp_terms *h = new rp_terms;
     h->prec = prec;
     h->len = len;
-    h->tile_bits = choose_tile_bits(len);
-    st = prec == RP_C64 ? upload_impl<float>(h, bits, coeff, nterms)
-                        : upload_impl<double>(h, bits, coeff, nterms);
+    h->acc64 = (prec == RP_C128 || !(flags & RP_TERMS_ACC_NATIVE)) ? 1 : 0;
+    choose_tiling(prec, len, &h->tile_bits, &h->threads);
+    if (prec == RP_C128) st = upload_impl<double, double>(h, bits, coeff, nterms);
+    else if (h->acc64) st = upload_impl<float, double>(h, bits, coeff, nterms);
+    else st = upload_impl<float, float>(h, bits, coeff, nterms);
     if (st) {
         free_handle(h);
         return st;
     }
     *out = h;
     return RP_OK;
+}
+
+int rp_terms_upload(rp_terms_t *out, int prec, int len, const uint64_t *bits,
+                    const double *coeff, uint64_t nterms)
+{
+    return rp_terms_upload_ex(out, prec, len, bits, coeff, nterms, 0);
 }
 
 int rp_terms_free(rp_terms_t h)
@@ -361,8 +568,9 @@ int rp_apply_ops(rp_terms_t h, void *res, const void *vec, int accumulate)
     RP_REQUIRE(h, "null term handle");
     RP_REQUIRE(res && vec, "null vector");
     RP_REQUIRE(res != vec, "res and vec must be different buffers");
-    return h->prec == RP_C64 ? apply_impl<float>(h, res, vec, accumulate)
-                             : apply_impl<double>(h, res, vec, accumulate);
+    if (h->prec == RP_C128) return apply_impl<double, double>(h, res, vec, accumulate);
+    if (h->acc64) return apply_impl<float, double>(h, res, vec, accumulate);
+    return apply_impl<float, float>(h, res, vec, accumulate);
 }
 
 }  // extern "C"

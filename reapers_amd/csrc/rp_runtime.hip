@@ -106,9 +106,9 @@ int rp_init(int device)
     c.name = prop.gcnArchName;
     // default-priority stream, not synchronizing with the legacy null stream
     RP_CHECK_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
-    // partials for up to 64 complex FP64 values per reduction block, plus
-    // room for the reduced results
-    c.work_bytes = (size_t)kRedBlocks * 64 * 16 + 64 * 1024;
+    // partials for up to 64 complex FP64 values and one norm per reduction
+    // block, plus room for the reduced results
+    c.work_bytes = (size_t)kRedBlocks * (2 * 64 + 1) * sizeof(double) + 64 * 1024;
     RP_CHECK_HIP(hipMalloc(&c.work, c.work_bytes));
     c.pinned_doubles = 4096;
     RP_CHECK_HIP(hipHostMalloc((void **)&c.pinned, c.pinned_doubles * sizeof(double),

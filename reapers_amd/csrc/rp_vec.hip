@@ -29,13 +29,13 @@ static inline int grid_for(uint64_t units, int threads, int cap)
 
 constexpr int kEwThreads = 256;
 constexpr int kEwMaxBlocks = 65536;
-constexpr int kMaxVecs = 64;      // vectors per fused sweep
-constexpr int kDotChunk = 16;     // vectors whose dots one sweep accumulates
+constexpr int kMaxVecs = 64;      // vectors per fused sweep (API limit)
+constexpr int kDotChunk = 8;      // vectors whose dots one launch accumulates
+constexpr int kUpdChunk = 16;     // vectors one update / lincomb launch reads
 constexpr int kWaves = kRedThreads / 64;
 
-struct VecPtrs16 { const void *p[kDotChunk]; };
-struct VecPtrs64 { const void *p[kMaxVecs]; };
-struct Coefs64 { double c[2 * kMaxVecs]; };
+struct VecPtrs16 { const void *p[kUpdChunk]; };
+struct Coefs16 { double c[2 * kUpdChunk]; };
 
 // ---------------------------------------------------------------------------
 // elementwise kernels
@@ -192,89 +192,93 @@ k_reduce_cols(const double *__restrict__ partial, int nparts, int width, double 
     if (threadIdx.x == 0) out[j] = sh[kWaves];
 }
 
-// <V[q]|w> for q < kc (kc <= kDotChunk), FP64 partials per block written to
-// partial[block*width + 2*(q0+q) + {0,1}]
-template <typename R>
+// <V[q]|w> for the KC vectors of one chunk, FP64 partials per block written
+// to partial[block*width + 2*(q0+q) + {0,1}].  All KC loads of an element are
+// issued before any use.
+template <typename R, int KC>
 __global__ void __launch_bounds__(kRedThreads)
-k_dots(VecPtrs16 V, int kc, const typename Cx<R>::T *__restrict__ w, uint64_t n,
+k_dots(VecPtrs16 V, const typename Cx<R>::T *__restrict__ w, uint64_t n,
        double *__restrict__ partial, int width, int q0)
 {
     using T = typename Cx<R>::T;
-    __shared__ double sh[(kWaves + 1) * 2 * kDotChunk];
-    double acc[2 * kDotChunk];
+    __shared__ double sh[(kWaves + 1) * 2 * KC];
+    double acc[2 * KC];
 #pragma unroll
-    for (int q = 0; q < 2 * kDotChunk; q++) acc[q] = 0.0;
+    for (int q = 0; q < 2 * KC; q++) acc[q] = 0.0;
     GRID_STRIDE(i, n) {
-        T y = w[i];
-        double yr = y.x, yi = y.y;
+        const T y = w[i];
+        T x[KC];
 #pragma unroll
-        for (int q = 0; q < kDotChunk; q++) {
-            if (q < kc) {
-                T x = ((const T *)V.p[q])[i];
-                double xr = x.x, xi = x.y;
-                acc[2 * q] += xr * yr + xi * yi;
-                acc[2 * q + 1] += xr * yi - xi * yr;
-            }
+        for (int q = 0; q < KC; q++) x[q] = ((const T *)V.p[q])[i];
+        const double yr = y.x, yi = y.y;
+#pragma unroll
+        for (int q = 0; q < KC; q++) {
+            const double xr = x[q].x, xi = x[q].y;
+            acc[2 * q] += xr * yr + xi * yi;
+            acc[2 * q + 1] += xr * yi - xi * yr;
         }
     }
-    block_reduce<2 * kDotChunk>(acc, sh);
-    if (threadIdx.x < 2 * kc)
+    block_reduce<2 * KC>(acc, sh);
+    if (threadIdx.x < 2 * KC)
         partial[(size_t)blockIdx.x * width + 2 * q0 + threadIdx.x] =
-            sh[kWaves * 2 * kDotChunk + threadIdx.x];
+            sh[kWaves * 2 * KC + threadIdx.x];
 }
 
-// w -= sum_q h[q] V[q] (FP64 arithmetic, one rounding), and ||w||^2 partials
-template <typename R>
+// w -= sum_q h[q0+q] V[q] for the KC vectors of one chunk (FP64 arithmetic,
+// one rounding); the last chunk also writes ||w||^2 partials
+template <typename R, int KC>
 __global__ void __launch_bounds__(kRedThreads)
-k_gs_update(VecPtrs64 V, int k, const double *__restrict__ h, typename Cx<R>::T *__restrict__ w,
+k_gs_update(VecPtrs16 V, const double *__restrict__ h, typename Cx<R>::T *__restrict__ w,
             uint64_t n, double *__restrict__ partial)
 {
     using T = typename Cx<R>::T;
-    __shared__ double hs[2 * kMaxVecs];
+    __shared__ double hs[2 * KC];
     __shared__ double sh[(kWaves + 1) * 1];
-    if (threadIdx.x < 2 * k) hs[threadIdx.x] = h[threadIdx.x];
+    if (threadIdx.x < 2 * KC) hs[threadIdx.x] = h[threadIdx.x];
     __syncthreads();
     double nacc[1] = {0.0};
     GRID_STRIDE(i, n) {
-        T y = w[i];
+        const T y = w[i];
+        T x[KC];
+#pragma unroll
+        for (int q = 0; q < KC; q++) x[q] = ((const T *)V.p[q])[i];
         double yr = y.x, yi = y.y;
 #pragma unroll
-        for (int q = 0; q < kMaxVecs; q++) {
-            if (q < k) {
-                T x = ((const T *)V.p[q])[i];
-                double xr = x.x, xi = x.y, hr = hs[2 * q], hi = hs[2 * q + 1];
-                yr -= hr * xr - hi * xi;
-                yi -= hr * xi + hi * xr;
-            }
+        for (int q = 0; q < KC; q++) {
+            const double xr = x[q].x, xi = x[q].y, hr = hs[2 * q], hi = hs[2 * q + 1];
+            yr -= hr * xr - hi * xi;
+            yi -= hr * xi + hi * xr;
         }
         T o;
         o.x = (R)yr;
         o.y = (R)yi;
         w[i] = o;
-        double orr = o.x, oi = o.y;
+        const double orr = o.x, oi = o.y;
         nacc[0] += orr * orr + oi * oi;
     }
-    block_reduce<1>(nacc, sh);
-    if (threadIdx.x == 0) partial[blockIdx.x] = sh[kWaves];
+    if (partial) {
+        block_reduce<1>(nacc, sh);
+        if (threadIdx.x == 0) partial[blockIdx.x] = sh[kWaves];
+    }
 }
 
-// y += sum_q c[q] V[q] (FP64 arithmetic, one rounding)
-template <typename R>
+// y += sum_q c[q] V[q] for the KC vectors of one chunk (FP64 arithmetic)
+template <typename R, int KC>
 __global__ void __launch_bounds__(kRedThreads)
-k_lincomb(VecPtrs64 V, Coefs64 c, int k, typename Cx<R>::T *__restrict__ y, uint64_t n)
+k_lincomb(VecPtrs16 V, Coefs16 c, typename Cx<R>::T *__restrict__ y, uint64_t n)
 {
     using T = typename Cx<R>::T;
     GRID_STRIDE(i, n) {
         T o = y[i];
+        T x[KC];
+#pragma unroll
+        for (int q = 0; q < KC; q++) x[q] = ((const T *)V.p[q])[i];
         double yr = o.x, yi = o.y;
 #pragma unroll
-        for (int q = 0; q < kMaxVecs; q++) {
-            if (q < k) {
-                T x = ((const T *)V.p[q])[i];
-                double xr = x.x, xi = x.y, cr = c.c[2 * q], ci = c.c[2 * q + 1];
-                yr += cr * xr - ci * xi;
-                yi += cr * xi + ci * xr;
-            }
+        for (int q = 0; q < KC; q++) {
+            const double xr = x[q].x, xi = x[q].y, cr = c.c[2 * q], ci = c.c[2 * q + 1];
+            yr += cr * xr - ci * xi;
+            yi += cr * xi + ci * xr;
         }
         o.x = (R)yr;
         o.y = (R)yi;
@@ -319,11 +323,12 @@ k_dot_partial(const typename Cx<R>::T *__restrict__ a, const typename Cx<R>::T *
 // ---------------------------------------------------------------------------
 // host helpers
 // ---------------------------------------------------------------------------
-// workspace layout: [0, kRedBlocks*2*kMaxVecs) partials, then results
 static inline double *ws_partial() { return (double *)ctx().work; }
+// [0, B*2*kMaxVecs): dot partials; [B*2*kMaxVecs, B*(2*kMaxVecs+1)): norm
+// partials; then the reduced results (B = kRedBlocks)
 static inline double *ws_result()
 {
-    return (double *)ctx().work + (size_t)kRedBlocks * 2 * kMaxVecs;
+    return (double *)ctx().work + (size_t)kRedBlocks * (2 * kMaxVecs + 1);
 }
 
 static inline int red_blocks(uint64_t n)
@@ -367,38 +372,87 @@ static int nrm2_launch(const void *a, uint64_t n, double *dst_dev)
     return RP_OK;
 }
 
+// launch kernel K<R, KC> for a runtime chunk width kc in [1, 16]
+#define RP_KC_SWITCH(kc, LAUNCH)                                                  \
+    switch (kc) {                                                                 \
+    case 1: LAUNCH(1); break;   case 2: LAUNCH(2); break;                         \
+    case 3: LAUNCH(3); break;   case 4: LAUNCH(4); break;                         \
+    case 5: LAUNCH(5); break;   case 6: LAUNCH(6); break;                         \
+    case 7: LAUNCH(7); break;   case 8: LAUNCH(8); break;                         \
+    case 9: LAUNCH(9); break;   case 10: LAUNCH(10); break;                       \
+    case 11: LAUNCH(11); break; case 12: LAUNCH(12); break;                       \
+    case 13: LAUNCH(13); break; case 14: LAUNCH(14); break;                       \
+    case 15: LAUNCH(15); break; default: LAUNCH(16); break;                       \
+    }
+
 template <typename R>
 static int gs_pass_impl(const void *const *V, int k, void *w, uint64_t n, double *h,
                         double *norm)
 {
     using T = typename Cx<R>::T;
-    int nb = red_blocks(n);
+    const int nb = red_blocks(n);
     const int width = 2 * kMaxVecs;
-    double *res = ws_result();          // res[0..2k): h, res[2k]: ||w||^2
+    double *res = ws_result();                    // res[0..2k): h, res[2k]: ||w||^2
+    double *npart = ws_partial() + (size_t)kRedBlocks * width;   // norm partials
     if (k > 0) {
         for (int q0 = 0; q0 < k; q0 += kDotChunk) {
-            VecPtrs16 P;
-            int kc = k - q0 < kDotChunk ? k - q0 : kDotChunk;
-            for (int q = 0; q < kDotChunk; q++) P.p[q] = q < kc ? V[q0 + q] : nullptr;
-            RP_LAUNCH(RP_KERNEL_GS, k_dots<R>, dim3(nb), dim3(kRedThreads), 0, P, kc,
-                      (const T *)w, n, ws_partial(), width, q0);
+            const int kc = k - q0 < kDotChunk ? k - q0 : kDotChunk;
+            VecPtrs16 P{};
+            for (int q = 0; q < kc; q++) P.p[q] = V[q0 + q];
+#define RP_DOTS(KC)                                                               \
+    RP_LAUNCH(RP_KERNEL_GS, (k_dots<R, KC>), dim3(nb), dim3(kRedThreads), 0, P, (const T *)w, n, \
+              ws_partial(), width, q0)
+            RP_KC_SWITCH(kc, RP_DOTS)
+#undef RP_DOTS
         }
         RP_LAUNCH(RP_KERNEL_GS, k_reduce_cols, dim3(2 * k), dim3(kRedThreads), 0,
                   (const double *)ws_partial(), nb, width, res);
     }
-    VecPtrs64 P;
-    for (int q = 0; q < kMaxVecs; q++) P.p[q] = q < k ? V[q] : nullptr;
-    // the update kernel writes its norm partials after the h reduction read
-    // them, so the partial area can be reused
-    RP_LAUNCH(RP_KERNEL_GS, k_gs_update<R>, dim3(nb), dim3(kRedThreads), 0, P, k,
-              (const double *)res, (T *)w, n, ws_partial() + (size_t)kRedBlocks * width / 2);
-    RP_LAUNCH(RP_KERNEL_GS, k_reduce_cols, dim3(1), dim3(kRedThreads), 0,
-              (const double *)(ws_partial() + (size_t)kRedBlocks * width / 2), nb, 1, res + 2 * k);
+    if (k == 0) {
+        int st = nrm2_launch<R>(w, n, res);
+        if (st) return st;
+    } else {
+        for (int q0 = 0; q0 < k; q0 += kUpdChunk) {
+            const int kc = k - q0 < kUpdChunk ? k - q0 : kUpdChunk;
+            const bool last = q0 + kc == k;
+            VecPtrs16 P{};
+            for (int q = 0; q < kc; q++) P.p[q] = V[q0 + q];
+#define RP_UPD(KC)                                                                \
+    RP_LAUNCH(RP_KERNEL_GS, (k_gs_update<R, KC>), dim3(nb), dim3(kRedThreads), 0, P,          \
+              (const double *)(res + 2 * q0), (T *)w, n, last ? npart : (double *)nullptr)
+            RP_KC_SWITCH(kc, RP_UPD)
+#undef RP_UPD
+        }
+        RP_LAUNCH(RP_KERNEL_GS, k_reduce_cols, dim3(1), dim3(kRedThreads), 0,
+                  (const double *)npart, nb, 1, res + 2 * k);
+    }
     double tmp[2 * kMaxVecs + 1];
     int st = fetch_results(tmp, 2 * k + 1);
     if (st) return st;
     for (int q = 0; q < 2 * k; q++) h[q] = tmp[q];
     *norm = sqrt(tmp[2 * k]);
+    return RP_OK;
+}
+
+template <typename R>
+static int lincomb_impl(void *y, const void *const *V, int k, const double *c, uint64_t n)
+{
+    using T = typename Cx<R>::T;
+    const int nb = red_blocks(n);
+    for (int q0 = 0; q0 < k; q0 += kUpdChunk) {
+        const int kc = k - q0 < kUpdChunk ? k - q0 : kUpdChunk;
+        VecPtrs16 P{};
+        Coefs16 C{};
+        for (int q = 0; q < kc; q++) {
+            P.p[q] = V[q0 + q];
+            C.c[2 * q] = c[2 * (q0 + q)];
+            C.c[2 * q + 1] = c[2 * (q0 + q) + 1];
+        }
+#define RP_LC(KC)                                                                 \
+    RP_LAUNCH(RP_KERNEL_OTHER, (k_lincomb<R, KC>), dim3(nb), dim3(kRedThreads), 0, P, C, (T *)y, n)
+        RP_KC_SWITCH(kc, RP_LC)
+#undef RP_LC
+    }
     return RP_OK;
 }
 
@@ -597,22 +651,9 @@ int rp_lincomb(int prec, void *y, const void *const *V, int k, const double *c, 
     RP_REQUIRE(k >= 0 && k <= kMaxVecs, "k must be in [0, 64]");
     if (n == 0 || k == 0) return RP_OK;
     RP_REQUIRE(y && V && c, "null argument");
-    VecPtrs64 P;
-    Coefs64 C;
-    for (int q = 0; q < kMaxVecs; q++) {
-        P.p[q] = q < k ? V[q] : nullptr;
-        C.c[2 * q] = q < k ? c[2 * q] : 0.0;
-        C.c[2 * q + 1] = q < k ? c[2 * q + 1] : 0.0;
-    }
     for (int q = 0; q < k; q++) RP_REQUIRE(V[q], "null basis vector");
-    int nb = red_blocks(n);
-    if (prec == RP_C64)
-        RP_LAUNCH(RP_KERNEL_OTHER, k_lincomb<float>, dim3(nb), dim3(kRedThreads), 0, P, C, k,
-                  (float2 *)y, n);
-    else
-        RP_LAUNCH(RP_KERNEL_OTHER, k_lincomb<double>, dim3(nb), dim3(kRedThreads), 0, P, C, k,
-                  (double2 *)y, n);
-    return RP_OK;
+    return prec == RP_C64 ? lincomb_impl<float>(y, V, k, c, n)
+                          : lincomb_impl<double>(y, V, k, c, n);
 }
 
 }  // extern "C"

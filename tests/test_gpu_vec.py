@@ -1,0 +1,126 @@
+"""GPU parity of the vector operations behind the C ABI (GPUImpl's BLAS-1,
+inc/gpuimpl.h:806-1062, and the fused Gram-Schmidt / linear-combination
+sweeps) against numpy in FP64.  Reductions accumulate in FP64 for both
+precisions, so FP32 results are checked to FP32 input rounding (1e-6) and
+FP64 results to 1e-13."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import reapers_amd as ra
+from gpu_util import DeviceBuffer, prec_of
+
+pytestmark = pytest.mark.gpu
+
+
+def _rand(rng, n, dtype):
+    return (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(dtype)
+
+
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+def test_blas1(gpu, dtype):
+    rng = np.random.default_rng(0)
+    n = 100003          # odd, not a power of two
+    a, b = _rand(rng, n, dtype), _rand(rng, n, dtype)
+    p = prec_of(dtype)
+    h = gpu.hip()
+    da, db = DeviceBuffer.from_array(a), DeviceBuffer.from_array(b)
+    re, im = ctypes.c_double(), ctypes.c_double()
+    ra.check(h.rp_vec_prod(p, da.ptr, db.ptr, n, ctypes.byref(re), ctypes.byref(im)))
+    ref = np.vdot(a.astype(complex), b.astype(complex))
+    assert abs(complex(re.value, im.value) - ref) <= 1e-12 * np.sqrt(n) * 4
+    nrm = ctypes.c_double()
+    ra.check(h.rp_vec_norm(p, da.ptr, n, ctypes.byref(nrm)))
+    assert abs(nrm.value - np.linalg.norm(a.astype(complex))) <= 1e-12 * nrm.value
+    tol = 1e-6 if dtype == np.complex64 else 1e-14
+    s = 0.3 - 1.7j
+    ra.check(h.rp_axpy(p, da.ptr, s.real, s.imag, db.ptr, n))
+    got = da.to_array(dtype, n)
+    assert np.abs(got - (a.astype(complex) + s * b.astype(complex))).max() <= tol * 10
+    ra.check(h.rp_scale_vec(p, da.ptr, 0.5, 0.25, n))
+    got2 = da.to_array(dtype, n)
+    assert np.abs(got2 - got.astype(complex) * (0.5 + 0.25j)).max() <= tol * 10
+    ra.check(h.rp_add_vec(p, da.ptr, db.ptr, n))
+    got3 = da.to_array(dtype, n)
+    assert np.abs(got3 - (got2.astype(complex) + b)).max() <= tol * 10
+    dc = DeviceBuffer(a.nbytes)
+    ra.check(h.rp_add_vec3(p, dc.ptr, da.ptr, db.ptr, n))
+    assert np.abs(dc.to_array(dtype, n) - (got3.astype(complex) + b)).max() <= tol * 10
+    ra.check(h.rp_zero_vec(p, dc.ptr, n))
+    assert np.all(dc.to_array(dtype, n) == 0)
+
+
+def test_precision_conversion_rounding(gpu):
+    rng = np.random.default_rng(1)
+    n = 4097
+    x = rng.standard_normal(n) * 1e3 + 1j * rng.standard_normal(n)
+    x = x.astype(np.complex128)
+    dx = DeviceBuffer.from_array(x)
+    df = DeviceBuffer(n * 8)
+    h = gpu.hip()
+    ra.check(h.rp_copy_vec(ra.RP_C64, df.ptr, ra.RP_C128, dx.ptr, n, ra.RP_ROUND_NEAREST))
+    assert np.array_equal(df.to_array(np.complex64, n), x.astype(np.complex64))
+    ra.check(h.rp_copy_vec(ra.RP_C64, df.ptr, ra.RP_C128, dx.ptr, n, ra.RP_ROUND_TOWARD_ZERO))
+    got = df.to_array(np.complex64, n)
+    for part in (np.real, np.imag):
+        g, r = part(got).astype(np.float64), part(x)
+        assert np.all(np.abs(g) <= np.abs(r))                 # never rounds away from zero
+        nxt = np.nextafter(part(got), np.sign(r).astype(np.float32) * np.inf).astype(np.float64)
+        assert np.all(np.abs(nxt) > np.abs(r) - 0) & True     # truncation is within one ulp
+        assert np.all(np.abs(g - r) < np.abs(nxt - g) + 1e-300)
+    dd = DeviceBuffer(n * 16)
+    ra.check(h.rp_copy_vec(ra.RP_C128, dd.ptr, ra.RP_C64, df.ptr, n, 0))
+    assert np.array_equal(dd.to_array(np.complex128, n), got.astype(np.complex128))
+
+
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+@pytest.mark.parametrize("k", [0, 1, 7, 8, 9, 16, 17, 20, 33, 64])
+def test_gs_pass_and_lincomb(gpu, dtype, k):
+    rng = np.random.default_rng(100 + k)
+    n = 65536 + 3
+    V = [_rand(rng, n, dtype) for _ in range(k)]
+    w = _rand(rng, n, dtype)
+    p = prec_of(dtype)
+    dV = [DeviceBuffer.from_array(v) for v in V]
+    dw = DeviceBuffer.from_array(w)
+    ptrs = (ctypes.c_void_p * max(k, 1))(*[d.ptr.value for d in dV])
+    hbuf = np.zeros(2 * max(k, 1))
+    nrm = ctypes.c_double()
+    ra.check(gpu.hip().rp_gs_pass(p, ptrs, k, dw.ptr, n, hbuf.ctypes.data_as(ctypes.c_void_p),
+                                  ctypes.byref(nrm)))
+    Vd = [v.astype(complex) for v in V]
+    wd = w.astype(complex)
+    href = np.array([np.vdot(v, wd) for v in Vd]) if k else np.zeros(0, complex)
+    hgot = hbuf[0:2 * k:2] + 1j * hbuf[1:2 * k:2]
+    if k:
+        assert np.abs(hgot - href).max() <= 1e-12 * np.sqrt(n) * 8
+    wref = wd - sum((href[i] * Vd[i] for i in range(k)), np.zeros(n, complex))
+    wgot = dw.to_array(dtype, n)
+    tol = 2e-6 * (1 + np.abs(href).sum()) if dtype == np.complex64 else 1e-12 * (1 + np.abs(href).sum())
+    assert np.abs(wgot - wref).max() <= tol
+    assert abs(nrm.value - np.linalg.norm(wgot.astype(complex))) <= 1e-12 * nrm.value
+    # lincomb: y += sum c_i V_i
+    if k:
+        c = rng.standard_normal(k) + 1j * rng.standard_normal(k)
+        cc = np.empty(2 * k); cc[0::2] = c.real; cc[1::2] = c.imag
+        y = _rand(rng, n, dtype)
+        dy = DeviceBuffer.from_array(y)
+        ra.check(gpu.hip().rp_lincomb(p, dy.ptr, ptrs, k, cc.ctypes.data_as(ctypes.c_void_p), n))
+        yref = y.astype(complex) + sum(c[i] * Vd[i] for i in range(k))
+        tol = 2e-6 * (1 + np.abs(c).sum()) if dtype == np.complex64 else 1e-12 * (1 + np.abs(c).sum())
+        assert np.abs(dy.to_array(dtype, n) - yref).max() <= tol
+
+
+def test_init_random_is_normalized_and_seeded(gpu):
+    n = 1 << 16
+    h = gpu.hip()
+    d1, d2 = DeviceBuffer(n * 16), DeviceBuffer(n * 16)
+    ra.check(h.rp_init_random(ra.RP_C128, d1.ptr, n, 42))
+    ra.check(h.rp_init_random(ra.RP_C128, d2.ptr, n, 42))
+    a, b = d1.to_array(np.complex128, n), d2.to_array(np.complex128, n)
+    assert np.array_equal(a, b)
+    assert abs(np.linalg.norm(a) - 1) < 1e-14
+    # Gaussian moments of the unnormalized components: mean 0, variance 1/(2n)
+    comp = np.concatenate([a.real, a.imag]) * np.sqrt(2 * n)
+    assert abs(comp.mean()) < 0.02 and abs(comp.var() - 1) < 0.03
