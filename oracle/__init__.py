@@ -15,6 +15,7 @@ LIB = os.path.join(HERE, "liboracle.so")
 _c = ctypes
 _vp = _c.c_void_p
 _lib = None
+_variants = {}
 
 
 class KrylovStats(_c.Structure):
@@ -22,12 +23,9 @@ class KrylovStats(_c.Structure):
                 ("breakdown", _c.c_int), ("last_nrm", _c.c_double)]
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB):
-            raise RuntimeError("oracle/liboracle.so is not built (make -C oracle)")
-        L = _c.CDLL(LIB)
+def _bind(L):
+    """Declare the C signatures on a loaded oracle library."""
+    if True:
         L.or_mt19937_kat.restype = _c.c_uint32
         L.or_mt_stream.argtypes = [_c.c_uint32, _vp, _c.c_int64]
         L.or_normal_stream_f32.argtypes = [_c.c_uint32, _vp, _c.c_int64]
@@ -67,8 +65,48 @@ def lib():
         L.or_lanczos_ground_state_f64.argtypes = [_c.c_int, _vp, _vp, _c.c_int64, _vp, _c.c_int,
                                                   _c.c_double, _c.c_int, _c.POINTER(_c.c_int)]
         L.or_num_threads.restype = _c.c_int
-        _lib = L
+        L.or_bernoulli_stream.argtypes = [_c.c_uint32, _c.c_double, _vp, _c.c_int64]
+        L.or_uniform_int_stream.argtypes = [_c.c_uint32, _c.c_int, _c.c_int, _vp, _c.c_int64]
+        L.or_canonical_stream.argtypes = [_c.c_uint32, _c.c_int, _vp, _c.c_int64]
+    return L
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError("oracle/liboracle.so is not built (make -C oracle)")
+        _lib = _bind(_c.CDLL(LIB))
     return _lib
+
+
+def planted(n):
+    """A deliberately wrong build of the oracle (OR_PLANT=n, see oracle.c),
+    built on demand; used only to show that the pinning tests catch it."""
+    if n not in _variants:
+        import subprocess
+        so = os.path.join(HERE, "liboracle_plant%d.so" % n)
+        subprocess.check_call(["make", "-s", "-C", HERE, "liboracle_plant%d.so" % n])
+        _variants[n] = _bind(_c.CDLL(so))
+    return _variants[n]
+
+
+class use:
+    """Context manager: route this module's wrappers to another loaded
+    oracle library (a planted variant)."""
+
+    def __init__(self, L):
+        self.L = L
+
+    def __enter__(self):
+        global _lib
+        self.prev = lib()
+        _lib = self.L
+        return self
+
+    def __exit__(self, *a):
+        global _lib
+        _lib = self.prev
 
 
 def _p(a):
@@ -95,6 +133,24 @@ def mt_stream(seed, n):
 def normal_stream(seed, n, fp=64):
     out = np.zeros(n, np.float32 if fp == 32 else np.float64)
     (lib().or_normal_stream_f32 if fp == 32 else lib().or_normal_stream_f64)(seed, _p(out), n)
+    return out
+
+
+def bernoulli_stream(seed, p, n):
+    out = np.zeros(n)
+    lib().or_bernoulli_stream(seed, p, _p(out), n)
+    return out
+
+
+def uniform_int_stream(seed, a, b, n):
+    out = np.zeros(n)
+    lib().or_uniform_int_stream(seed, a, b, _p(out), n)
+    return out
+
+
+def canonical_stream(seed, n, fp=64):
+    out = np.zeros(n)
+    lib().or_canonical_stream(seed, fp, _p(out), n)
     return out
 
 

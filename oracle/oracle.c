@@ -20,6 +20,14 @@
  */
 #include "oracle.h"
 
+/* OR_PLANT selects a deliberately wrong variant for tests that show the
+ * pinning checks catch it (0 = the restatement itself):
+ *   1 drop the product sign, 2 swap X and Z bits in apply,
+ *   3 drop the restart coupling beta in Hfull */
+#ifndef OR_PLANT
+#define OR_PLANT 0
+#endif
+
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -83,6 +91,8 @@ uint32_t or_mt19937_kat(void)
     for (int i = 0; i < 10000; i++) v = or_mt_next(&g);
     return v;
 }
+
+void or_uniform_int_stream(uint32_t seed, int a, int b, double *out, int64_t n);
 
 void or_mt_stream(uint32_t seed, uint32_t *out, int64_t n)
 {
@@ -172,6 +182,12 @@ void or_normal_stream_f32(uint32_t seed, float *out, int64_t n)
     for (int64_t i = 0; i < n; i++) out[i] = or_normal_draw_f32(&d, &g);
 }
 
+/* bernoulli(p), uniform_int(a, b) and generate_canonical streams from one
+ * engine, for comparing against libstdc++ */
+void or_bernoulli_stream(uint32_t seed, double p, double *out, int64_t n);
+void or_uniform_int_stream(uint32_t seed, int a, int b, double *out, int64_t n);
+void or_canonical_stream(uint32_t seed, int fp, double *out, int64_t n);
+
 void or_normal_stream_f64(uint32_t seed, double *out, int64_t n)
 {
     or_mt19937 g;
@@ -185,6 +201,21 @@ void or_normal_stream_f64(uint32_t seed, double *out, int64_t n)
 static int or_bernoulli(or_mt19937 *g, double p)
 {
     return (or_canonical_f64(g) - 0.0) < p * (1.0 - 0.0);
+}
+
+void or_bernoulli_stream(uint32_t seed, double p, double *out, int64_t n)
+{
+    or_mt19937 g;
+    or_mt_seed(&g, seed);
+    for (int64_t i = 0; i < n; i++) out[i] = or_bernoulli(&g, p) ? 1.0 : 0.0;
+}
+
+void or_canonical_stream(uint32_t seed, int fp, double *out, int64_t n)
+{
+    or_mt19937 g;
+    or_mt_seed(&g, seed);
+    for (int64_t i = 0; i < n; i++)
+        out[i] = fp == 32 ? (double)or_canonical_f32(&g) : or_canonical_f64(&g);
 }
 
 /* std::uniform_int_distribution<int>(a, b)(mt19937): Lemire's nearly
@@ -206,6 +237,13 @@ int or_uniform_int(or_mt19937 *g, int a, int b)
     return (int)(uint32_t)(product >> 32) + a;
 }
 
+void or_uniform_int_stream(uint32_t seed, int a, int b, double *out, int64_t n)
+{
+    or_mt19937 g;
+    or_mt_seed(&g, seed);
+    for (int64_t i = 0; i < n; i++) out[i] = or_uniform_int(&g, a, b);
+}
+
 /* ===================================================================== */
 /* Pauli strings (inc/ops.h)                                              */
 /* ===================================================================== */
@@ -216,6 +254,11 @@ static int or_site(uint64_t bits, int s) { return (int)((bits >> (2 * s)) & 3u);
 /* inc/ops.h:151-161: z.x, z.y, y.x, y.y carry a minus sign */
 static int or_prod_has_minus(int op0, int op1)
 {
+#if OR_PLANT == 1
+    /* planted mistake: the product never carries a minus sign */
+    (void)op0; (void)op1;
+    return 0;
+#endif
     switch ((op0 << 2) | op1) {
     case 0x9: case 0xb: case 0xd: case 0xf: return 1;
     default: return 0;
@@ -240,6 +283,10 @@ void or_pauli_apply(uint64_t bits, uint32_t n, uint32_t *res, int *minus)
 {
     int m = 0;
     uint64_t b = bits;
+#if OR_PLANT == 2
+    /* planted mistake: X and Z bits swapped */
+    b = ((bits & 0x5555555555555555ull) << 1) | ((bits >> 1) & 0x5555555555555555ull);
+#endif
     for (int i = 0; i < 32; i++) {
         if (b & 1u) n ^= (uint32_t)1u << i;
         b >>= 1;

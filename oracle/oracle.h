@@ -56,6 +56,9 @@ double   or_canonical_f64(or_mt19937 *g);
 void     or_normal_stream_f32(uint32_t seed, float *out, int64_t n);
 void     or_normal_stream_f64(uint32_t seed, double *out, int64_t n);
 int      or_uniform_int(or_mt19937 *g, int a, int b);
+void     or_bernoulli_stream(uint32_t seed, double p, double *out, int64_t n);
+void     or_uniform_int_stream(uint32_t seed, int a, int b, double *out, int64_t n);
+void     or_canonical_stream(uint32_t seed, int fp, double *out, int64_t n);
 
 /* ---- Pauli string algebra (inc/ops.h) ------------------------------- */
 /* product op0*op1: returns the bits, *minus set per inc/ops.h:319-327 */

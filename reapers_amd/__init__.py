@@ -110,6 +110,8 @@ def host():
             "rph_majorana": ([_c.c_int, _c.c_int, _c.c_int, _c.c_int, _vp, _vp], _c.c_int),
             "rph_ops_mul": ([_c.c_int, _vp, _vp, _i64, _vp, _vp, _i64, _vp, _vp, _i64,
                              _c.POINTER(_i64)], _c.c_int),
+            "rph_random_stream": ([_c.c_int, _c.c_int, _c.c_uint32, _dbl, _dbl, _vp, _i64],
+                                  _c.c_int),
             "rph_init": ([_c.c_int], _c.c_int),
             "rph_ham_new": ([_c.c_int, _c.c_int, _vp, _vp, _i64], _vp),
             "rph_ham_info": ([_vp, _c.POINTER(TermsInfo)], _c.c_int),
@@ -172,6 +174,16 @@ def majorana(N, n, fp=32, std_gamma=True):
     co = np.zeros(2, np.float64)
     hcheck(host().rph_majorana(fp, N, n, int(std_gamma), _ptr(bits), _ptr(co)))
     return int(bits[0]), complex(co[0], co[1])
+
+
+def random_stream(kind, n, seed, fp=64, a=0.0, b=0.0):
+    """n draws from std::mt19937(seed) through libstdc++ <random>:
+    kind 'normal' (normal_distribution<fp>(0,1)), 'bernoulli' (p=a),
+    'uniform_int' (a..b) or 'canonical' (generate_canonical<fp>)."""
+    k = {"normal": 0, "bernoulli": 1, "uniform_int": 2, "canonical": 3}[kind]
+    out = np.zeros(n)
+    hcheck(host().rph_random_stream(k, fp, seed, a, b, _ptr(out), n))
+    return out
 
 
 def ops_mul(a, b, fp=64):

@@ -136,6 +136,46 @@ int rph_ops_mul(int fp, const uint64_t *b0, const double *c0, int64_t n0, const 
     });
 }
 
+// raw <random> streams from std::mt19937(seed), as the SYK generators draw
+// them (for comparing against other implementations of the same algorithms)
+int rph_random_stream(int kind, int fp, uint32_t seed, double param_a, double param_b,
+                      double *out, int64_t n)
+{
+    return guarded([&] {
+        std::mt19937 g(seed);
+        switch (kind) {
+        case 0: {  // normal(0, 1)
+            if (fp == 32) {
+                std::normal_distribution<float> d(0.0, 1.0);
+                for (int64_t i = 0; i < n; i++) out[i] = d(g);
+            } else {
+                std::normal_distribution<double> d(0.0, 1.0);
+                for (int64_t i = 0; i < n; i++) out[i] = d(g);
+            }
+            break;
+        }
+        case 1: {  // bernoulli(param_a)
+            std::bernoulli_distribution d(param_a);
+            for (int64_t i = 0; i < n; i++) out[i] = d(g) ? 1.0 : 0.0;
+            break;
+        }
+        case 2: {  // uniform_int(param_a, param_b)
+            std::uniform_int_distribution<int> d((int)param_a, (int)param_b);
+            for (int64_t i = 0; i < n; i++) out[i] = d(g);
+            break;
+        }
+        case 3: {  // generate_canonical
+            for (int64_t i = 0; i < n; i++)
+                out[i] = fp == 32 ? (double)std::generate_canonical<float, 24>(g)
+                                  : std::generate_canonical<double, 53>(g);
+            break;
+        }
+        default:
+            ThrowException(InvalidArgument, "kind", "must be 0..3");
+        }
+    });
+}
+
 int rph_init(int device)
 {
     return guarded([&] {
