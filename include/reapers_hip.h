@@ -167,6 +167,31 @@ int rp_prof_enable(int on);
 int rp_prof_reset(void);
 int rp_prof_query(int kernel_class, uint64_t *launches, double *total_ms);
 
+/* --- Krylov-schedule proposer network (no counterpart in the reference,
+ * whose Krylov::evolve uses one fixed dimension for all restarts,
+ * inc/algo.h:266-269).  A 12 -> 64 -> 64 -> 2 FP32 perceptron with tanh
+ * hidden layers; parameters are one array of RP_MLP_NPARAMS floats laid out
+ * W1[64][12], b1[64], W2[64][64], b2[64], W3[2][64], b3[2].
+ * Training and inference run on the library stream; all pointers except
+ * the infer call's x_host / y_host and loss are device pointers. */
+#define RP_MLP_IN 12
+#define RP_MLP_HIDDEN 64
+#define RP_MLP_OUT 2
+#define RP_MLP_NPARAMS 5122
+#define RP_MLP_MAX_INFER 64
+/* Y[n][2] = net(X[n][12]) */
+int rp_mlp_forward(const float *params, const float *X, int64_t n, float *Y);
+/* loss = sum_{s,o} W[s][o] (net(X[s])_o - T[s][o])^2 / n (W may be null:
+ * all ones) and grad = d loss / d params; deterministic for a given n */
+int rp_mlp_loss_grad(const float *params, const float *X, const float *T, const float *W,
+                     int64_t n, float *grad, double *loss);
+/* one AdamW step (decoupled weight decay), step counts from 1 */
+int rp_mlp_adam(float *params, const float *grad, float *m, float *v, int64_t nparams,
+                double lr, double beta1, double beta2, double eps, double weight_decay,
+                int64_t step);
+/* forward of n <= RP_MLP_MAX_INFER host samples; blocks until y_host is set */
+int rp_mlp_infer(const float *params, const float *x_host, int n, float *y_host);
+
 #ifdef __cplusplus
 }
 #endif

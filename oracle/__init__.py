@@ -68,6 +68,12 @@ def _bind(L):
         L.or_bernoulli_stream.argtypes = [_c.c_uint32, _c.c_double, _vp, _c.c_int64]
         L.or_uniform_int_stream.argtypes = [_c.c_uint32, _c.c_int, _c.c_int, _vp, _c.c_int64]
         L.or_canonical_stream.argtypes = [_c.c_uint32, _c.c_int, _vp, _c.c_int64]
+        L.or_mlp_nparams.restype = _c.c_int
+        L.or_mlp_forward_f64.argtypes = [_vp, _vp, _c.c_int64, _vp]
+        L.or_mlp_loss_grad_f64.restype = _c.c_double
+        L.or_mlp_loss_grad_f64.argtypes = [_vp, _vp, _vp, _vp, _c.c_int64, _vp]
+        L.or_mlp_adam_f64.argtypes = [_vp, _vp, _vp, _vp, _c.c_int64, _c.c_double, _c.c_double,
+                                      _c.c_double, _c.c_double, _c.c_double, _c.c_int64]
     return L
 
 
@@ -290,3 +296,35 @@ def lanczos_ground_state(length, bits, coeff, psi, krydim=5, eps=1e-10, maxiter=
 
 def num_threads():
     return lib().or_num_threads()
+
+
+# ---- Krylov-schedule proposer network ------------------------------------
+MLP_NPARAMS = 5122
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def mlp_forward(params, X):
+    """Y[n][2] of the 12-64-64-2 tanh network, in double precision."""
+    p, X = _f64(params), _f64(X).reshape(-1, 12)
+    Y = np.empty((X.shape[0], 2))
+    lib().or_mlp_forward_f64(_p(p), _p(X), X.shape[0], _p(Y))
+    return Y
+
+
+def mlp_loss_grad(params, X, T, W=None):
+    """(loss, gradient) of sum w (y - t)^2 / n."""
+    p, X, T = _f64(params), _f64(X).reshape(-1, 12), _f64(T).reshape(-1, 2)
+    Wc = None if W is None else _f64(W).reshape(-1, 2)
+    g = np.empty(MLP_NPARAMS)
+    loss = lib().or_mlp_loss_grad_f64(_p(p), _p(X), _p(T), None if Wc is None else _p(Wc),
+                                      X.shape[0], _p(g))
+    return loss, g
+
+
+def mlp_adam(params, grad, m, v, lr, b1, b2, eps, wd, step):
+    """One AdamW step in place on float64 arrays params, m, v."""
+    lib().or_mlp_adam_f64(_p(params), _p(_f64(grad)), _p(m), _p(v), params.size, lr, b1, b2,
+                          eps, wd, step)

@@ -130,6 +130,16 @@ double or_lanczos_ground_state_f64(int len, const uint64_t *bits, const double *
 
 int or_num_threads(void);
 
+/* ---- Krylov-schedule proposer network (oracle_mlp.c) ----------------- */
+int    or_mlp_nparams(void);
+void   or_mlp_forward_f64(const double *p, const double *X, int64_t n, double *Y);
+/* returns the loss; g <- its gradient (W may be null: all ones) */
+double or_mlp_loss_grad_f64(const double *p, const double *X, const double *T,
+                            const double *W, int64_t n, double *g);
+void   or_mlp_adam_f64(double *p, const double *g, double *m, double *v, int64_t np,
+                       double lr, double b1, double b2, double eps, double wd,
+                       int64_t step);
+
 #ifdef __cplusplus
 }
 #endif

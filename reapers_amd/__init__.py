@@ -45,7 +45,8 @@ class TermsInfo(_c.Structure):
 class EvolveStats(_c.Structure):
     _fields_ = [("matvecs", _i64), ("restarts", _i64), ("gs_passes", _i64),
                 ("breakdown", _c.c_int32), ("substeps", _c.c_int32),
-                ("last_nrm", _dbl), ("host_ms", _dbl), ("wall_ms", _dbl)]
+                ("last_nrm", _dbl), ("host_ms", _dbl), ("wall_ms", _dbl),
+                ("rejected", _c.c_int32), ("reserved", _c.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -86,6 +87,10 @@ def hip():
             "rp_prof_query": [_c.c_int, _c.POINTER(_u64), _c.POINTER(_dbl)],
             "rp_mem_get_info": [_c.POINTER(_c.c_size_t), _c.POINTER(_c.c_size_t)],
             "rp_device_props": [_c.POINTER(_c.c_int), _c.c_char_p, _c.c_size_t],
+            "rp_mlp_forward": [_vp, _vp, _i64, _vp],
+            "rp_mlp_loss_grad": [_vp, _vp, _vp, _vp, _i64, _vp, _c.POINTER(_dbl)],
+            "rp_mlp_adam": [_vp, _vp, _vp, _vp, _i64, _dbl, _dbl, _dbl, _dbl, _dbl, _i64],
+            "rp_mlp_infer": [_vp, _vp, _c.c_int, _vp],
         }
         for name, args in sig.items():
             f = getattr(lib, name)
@@ -107,6 +112,8 @@ def host():
         sig = {
             "rph_syk_gen": ([_c.c_int, _c.c_int, _dbl, _dbl, _c.c_int, _c.c_uint32, _c.c_int,
                              _vp, _vp, _i64, _c.POINTER(_i64)], _c.c_int),
+            "rph_xxz_gen": ([_c.c_int, _c.c_int, _dbl, _dbl, _dbl, _c.c_uint32, _vp, _vp, _i64,
+                             _c.POINTER(_i64)], _c.c_int),
             "rph_majorana": ([_c.c_int, _c.c_int, _c.c_int, _c.c_int, _vp, _vp], _c.c_int),
             "rph_ops_mul": ([_c.c_int, _vp, _vp, _i64, _vp, _vp, _i64, _vp, _vp, _i64,
                              _c.POINTER(_i64)], _c.c_int),
@@ -125,6 +132,15 @@ def host():
             "rph_state_apply": ([_vp, _vp], _c.c_int),
             "rph_state_evolve": ([_vp, _vp, _dbl, _dbl, _c.c_int, _dbl, _c.c_int, _c.c_int,
                                   _dbl, _c.POINTER(EvolveStats)], _c.c_int),
+            "rph_state_evolve_list": ([_vp, _vp, _dbl, _dbl, _vp, _c.c_int, _c.c_int, _dbl,
+                                       _c.c_int, _c.c_int, _c.POINTER(EvolveStats)], _c.c_int),
+            "rph_sched_list_new": ([_vp, _c.c_int, _c.c_int], _vp),
+            "rph_sched_mlp_new": ([_vp, _c.c_int, _c.c_int, _c.c_int], _vp),
+            "rph_sched_free": ([_vp], None),
+            "rph_sched_set_tracing": ([_vp, _c.c_int], _c.c_int),
+            "rph_sched_trace": ([_vp, _vp, _vp, _vp, _c.c_int, _c.POINTER(_c.c_int)], _c.c_int),
+            "rph_state_evolve_sched": ([_vp, _vp, _vp, _dbl, _dbl, _c.c_int, _dbl, _c.c_int,
+                                        _c.c_int, _c.POINTER(EvolveStats)], _c.c_int),
             "rph_state_inner": ([_vp, _vp, _c.POINTER(_dbl), _c.POINTER(_dbl)], _c.c_int),
             "rph_state_norm": ([_vp, _c.POINTER(_dbl)], _c.c_int),
             "rph_state_ground": ([_vp, _vp, _c.c_int, _dbl, _c.c_int, _c.POINTER(_dbl)], _c.c_int),
@@ -167,6 +183,17 @@ def syk_terms(N, sp=0.0, J=1.0, seed=0, fp=32, std_gamma=True, regularize=False)
         if n.value >= 0:
             return bits[:n.value].copy(), (co[0:2 * n.value:2] + 1j * co[1:2 * n.value:2])
         cap = -n.value
+
+
+def xxz_terms(L, J=1.0, Delta=1.0, W=1.0, seed=0, fp=32):
+    """Term list of the random-field XXZ chain XXZ(L, J, Delta, W).gen_ham(
+    std::mt19937(seed))."""
+    cap = 4 * L
+    bits = np.zeros(cap, np.uint64)
+    co = np.zeros(2 * cap, np.float64)
+    n = _i64()
+    hcheck(host().rph_xxz_gen(fp, L, J, Delta, W, seed, _ptr(bits), _ptr(co), cap, _c.byref(n)))
+    return bits[:n.value].copy(), (co[0:2 * n.value:2] + 1j * co[1:2 * n.value:2])
 
 
 def majorana(N, n, fp=32, std_gamma=True):
@@ -280,6 +307,23 @@ class DeviceState:
                                        substep, _c.byref(st)))
         return st
 
+    def evolve_schedule(self, ham, t, dims, nsub=1, beta=0.0, tol=0.0, max_iters=0, reorth=0):
+        """Evolve with Krylov dimension dims[min(r, len-1)] at restart r and
+        nsub equal sub-steps."""
+        d = (_c.c_int * len(dims))(*dims)
+        st = EvolveStats()
+        hcheck(host().rph_state_evolve_list(self._s, ham._h, t, beta, d, len(dims), nsub, tol,
+                                            max_iters, reorth, _c.byref(st)))
+        return st
+
+    def evolve_with(self, ham, t, sched, beta=0.0, krydim=20, tol=0.0, max_iters=0, reorth=0):
+        """Evolve with a Schedule; krydim is the fixed dimension a rejected
+        proposal falls back to."""
+        st = EvolveStats()
+        hcheck(host().rph_state_evolve_sched(self._s, ham._h, sched._h, t, beta, krydim, tol,
+                                             max_iters, reorth, _c.byref(st)))
+        return st
+
     def inner(self, other):
         re, im = _dbl(), _dbl()
         hcheck(host().rph_state_inner(self._s, other._s, _c.byref(re), _c.byref(im)))
@@ -305,3 +349,122 @@ class DeviceState:
             self.close()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------------------
+# Krylov schedules and the proposer network
+# ---------------------------------------------------------------------------
+MLP_IN, MLP_OUT, MLP_NPARAMS, MLP_MAX_INFER = 12, 2, 5122, 64
+
+
+class Schedule:
+    """A C++ KrylovSchedule handle (ListSchedule or MLPSchedule) that can
+    record every decision with the proposer's 12 inputs."""
+
+    def __init__(self, handle):
+        if not handle:
+            raise RpError(-1, host().rph_last_error().decode())
+        self._h = handle
+
+    @classmethod
+    def from_list(cls, dims, nsub=1):
+        d = (_c.c_int * len(dims))(*dims)
+        return cls(host().rph_sched_list_new(d, len(dims), nsub))
+
+    @classmethod
+    def from_mlp(cls, params, min_dim=3, max_dim=40, max_sub=8):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        assert p.size == MLP_NPARAMS
+        return cls(host().rph_sched_mlp_new(_ptr(p), min_dim, max_dim, max_sub))
+
+    def tracing(self, on=True):
+        hcheck(host().rph_sched_set_tracing(self._h, 1 if on else 0))
+
+    def trace(self):
+        """(features[n,12] float32, meta[n,4] int32 (kind, value, restart,
+        sub-step), previous update norms[n])."""
+        cnt = _c.c_int()
+        hcheck(host().rph_sched_trace(self._h, None, None, None, 0, _c.byref(cnt)))
+        n = cnt.value
+        f = np.zeros((n, MLP_IN), np.float32)
+        m = np.zeros((n, 4), np.int32)
+        u = np.zeros(n)
+        hcheck(host().rph_sched_trace(self._h, _ptr(f), _ptr(m), _ptr(u), n, _c.byref(cnt)))
+        return f, m, u
+
+    def close(self):
+        if self._h:
+            host().rph_sched_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceArray:
+    """A float32 device array from rp_malloc (for the proposer's training)."""
+
+    def __init__(self, host_array):
+        a = np.ascontiguousarray(host_array, dtype=np.float32)
+        self.shape, self.nbytes = a.shape, a.nbytes
+        self.ptr = _vp()
+        check(hip().rp_malloc(_c.byref(self.ptr), max(4, a.nbytes)))
+        if a.nbytes:
+            check(hip().rp_memcpy(self.ptr, _ptr(a), a.nbytes, RP_H2D))
+
+    @classmethod
+    def zeros(cls, shape):
+        return cls(np.zeros(shape, np.float32))
+
+    def get(self):
+        a = np.empty(self.shape, np.float32)
+        check(hip().rp_memcpy(_ptr(a), self.ptr, self.nbytes, RP_D2H))
+        return a
+
+    def set(self, host_array):
+        a = np.ascontiguousarray(host_array, dtype=np.float32)
+        assert a.nbytes == self.nbytes
+        check(hip().rp_memcpy(self.ptr, _ptr(a), a.nbytes, RP_H2D))
+
+    def close(self):
+        if self.ptr:
+            hip().rp_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def mlp_forward(params, X):
+    """Y = net(X) by the HIP forward kernel; params and X are DeviceArrays."""
+    n = X.shape[0]
+    Y = DeviceArray.zeros((n, MLP_OUT))
+    check(hip().rp_mlp_forward(params.ptr, X.ptr, n, Y.ptr))
+    return Y
+
+
+def mlp_loss_grad(params, X, T, W, grad):
+    """loss of sum W (net(X) - T)^2 / n; grad (a DeviceArray) <- its gradient."""
+    loss = _dbl()
+    check(hip().rp_mlp_loss_grad(params.ptr, X.ptr, T.ptr, None if W is None else W.ptr,
+                                 X.shape[0], grad.ptr, _c.byref(loss)))
+    return loss.value
+
+
+def mlp_adam(params, grad, m, v, step, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8, wd=0.0):
+    check(hip().rp_mlp_adam(params.ptr, grad.ptr, m.ptr, v.ptr, MLP_NPARAMS, lr, b1, b2, eps,
+                            wd, step))
+
+
+def mlp_infer(params, x):
+    """net(x) for up to MLP_MAX_INFER host samples, through rp_mlp_infer."""
+    x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, MLP_IN)
+    y = np.empty((x.shape[0], MLP_OUT), np.float32)
+    check(hip().rp_mlp_infer(params.ptr, _ptr(x), x.shape[0], _ptr(y)))
+    return y

@@ -10,6 +10,8 @@
 
 #include <chrono>
 #include <cstdio>
+#include <cstring>
+#include <memory>
 #include <string>
 
 using namespace REAPERS;
@@ -82,6 +84,7 @@ typedef struct {
     int64_t matvecs, restarts, gs_passes;
     int32_t breakdown, substeps;
     double last_nrm, host_ms, wall_ms;
+    int32_t rejected, reserved;
 } rph_stats;
 
 const char *rph_last_error(void) { return g_err.c_str(); }
@@ -100,6 +103,22 @@ int rph_syk_gen(int fp, int N, double sp, double J, int std_gamma, uint32_t seed
             SYK<double, GPUImpl> syk(N, sp, J, std_gamma != 0);
             auto ham = syk.gen_ham(rg, regularize != 0);
             *nterms = export_ham<double>(ham, bits, coeff, cap);
+        }
+    });
+}
+
+// XXZ(L, J, Delta, W).gen_ham(std::mt19937(seed))
+int rph_xxz_gen(int fp, int L, double J, double Delta, double W, uint32_t seed, uint64_t *bits,
+                double *coeff, int64_t cap, int64_t *nterms)
+{
+    return guarded([&] {
+        std::mt19937 rg(seed);
+        if (fp == 32) {
+            XXZ<float, GPUImpl> m(L, (float)J, (float)Delta, (float)W);
+            *nterms = export_ham<float>(m.gen_ham(rg), bits, coeff, cap);
+        } else {
+            XXZ<double, GPUImpl> m(L, J, Delta, W);
+            *nterms = export_ham<double>(m.gen_ham(rg), bits, coeff, cap);
         }
     });
 }
@@ -286,36 +305,202 @@ int rph_state_apply(void *sp, void *hp)
     });
 }
 
+static void fill_stats(rph_stats *out, const KrylovStats &st, double wall)
+{
+    if (!out) return;
+    out->matvecs = st.matvecs;
+    out->restarts = st.restarts;
+    out->gs_passes = st.gs_passes;
+    out->breakdown = st.breakdown;
+    out->substeps = st.substeps;
+    out->last_nrm = st.last_nrm;
+    out->host_ms = st.host_ms;
+    out->wall_ms = wall;
+    out->rejected = st.rejected;
+    out->reserved = 0;
+}
+
 int rph_state_evolve(void *sp, void *hp, double t, double beta, int krydim, double tol,
-                     int max_iters, int reorth, double substep, rph_stats *out)
+                     int max_iters, int reorth, double substep, rph_stats *out);
+
+}  // extern "C"
+
+namespace {
+
+// A schedule handle: the schedule plus a record of every decision it made
+// (the proposer's 12 inputs, the decision's kind and value) when tracing.
+struct Sched : KrylovSchedule {
+    std::unique_ptr<KrylovSchedule> inner;
+    int max_dim = 64;
+    bool tracing = false;
+    struct Rec {
+        float f[RP_MLP_IN];
+        int kind, value, restart, substep;
+        double last_update;
+    };
+    std::vector<Rec> trace;
+
+    void record(const RestartInfo &ri, bool sub, int value) {
+        if (!tracing) return;
+        Rec r;
+        proposer_features(ri, sub, r.f);
+        r.kind = sub ? 0 : 1;
+        r.value = value;
+        r.restart = ri.restart;
+        r.substep = ri.substep;
+        r.last_update = ri.last_update;
+        trace.push_back(r);
+    }
+    int substeps(const RestartInfo &ri) override {
+        int v = inner->substeps(ri);
+        record(ri, true, v);
+        return v;
+    }
+    int krydim(const RestartInfo &ri) override {
+        int v = inner->krydim(ri);
+        record(ri, false, std::max(2, std::min(v, max_dim)));
+        return v;
+    }
+};
+
+template<typename F>
+int run_evolve(void *sp, void *hp, double t, double beta, KrylovOptions &opt, rph_stats *out,
+               F &&extra)
 {
     return guarded([&] {
         St *s = (St *)sp;
         Ham *h = (Ham *)hp;
         if (s->fp != h->fp) ThrowException(InvalidArgument, "precision of state and operator must match");
-        KrylovOptions opt;
-        opt.krydim = krydim;
-        opt.tol = tol;
-        opt.max_iters = max_iters;
-        opt.reorth = reorth ? Reorth::Always : Reorth::Selective;
-        opt.substep = substep;
+        extra(opt);
         auto t0 = std::chrono::steady_clock::now();
         KrylovStats st = s->fp == 32 ? s->f->evolve_ex(*h->f, t, beta, opt)
                                      : s->d->evolve_ex(*h->d, t, beta, opt);
         GPUImpl::synchronize();
         double wall = std::chrono::duration<double, std::milli>(
                           std::chrono::steady_clock::now() - t0).count();
-        if (out) {
-            out->matvecs = st.matvecs;
-            out->restarts = st.restarts;
-            out->gs_passes = st.gs_passes;
-            out->breakdown = st.breakdown;
-            out->substeps = st.substeps;
-            out->last_nrm = st.last_nrm;
-            out->host_ms = st.host_ms;
-            out->wall_ms = wall;
-        }
+        fill_stats(out, st, wall);
     });
+}
+
+}  // namespace
+
+extern "C" {
+
+int rph_state_evolve(void *sp, void *hp, double t, double beta, int krydim, double tol,
+                     int max_iters, int reorth, double substep, rph_stats *out)
+{
+    KrylovOptions opt;
+    opt.krydim = krydim;
+    opt.tol = tol;
+    opt.max_iters = max_iters;
+    opt.reorth = reorth ? Reorth::Always : Reorth::Selective;
+    opt.substep = substep;
+    return run_evolve(sp, hp, t, beta, opt, out, [](KrylovOptions &) {});
+}
+
+// evolve with a list schedule: Krylov dimension dims[min(r, ndims-1)] at
+// restart r, nsub equal sub-steps
+int rph_state_evolve_list(void *sp, void *hp, double t, double beta, const int *dims,
+                          int ndims, int nsub, double tol, int max_iters, int reorth,
+                          rph_stats *out)
+{
+    if (!dims || ndims < 1) {
+        g_err = "dims cannot be empty";
+        return RP_ERR_INVALID_ARGUMENT;
+    }
+    ListSchedule sched(std::vector<int>(dims, dims + ndims), nsub);
+    KrylovOptions opt;
+    opt.krydim = dims[ndims - 1];
+    opt.tol = tol;
+    opt.max_iters = max_iters;
+    opt.reorth = reorth ? Reorth::Always : Reorth::Selective;
+    opt.schedule = &sched;
+    opt.max_krydim = std::max(2, *std::max_element(dims, dims + ndims));
+    return run_evolve(sp, hp, t, beta, opt, out, [](KrylovOptions &) {});
+}
+
+// schedule handles --------------------------------------------------------
+
+void *rph_sched_list_new(const int *dims, int ndims, int nsub)
+{
+    Sched *h = nullptr;
+    int st = guarded([&] {
+        if (!dims || ndims < 1) ThrowException(InvalidArgument, "dims", "cannot be empty");
+        auto p = std::make_unique<Sched>();
+        p->inner = std::make_unique<ListSchedule>(std::vector<int>(dims, dims + ndims), nsub);
+        p->max_dim = std::max(2, *std::max_element(dims, dims + ndims));
+        h = p.release();
+    });
+    return st ? nullptr : h;
+}
+
+void *rph_sched_mlp_new(const float *params, int min_dim, int max_dim, int max_sub)
+{
+    Sched *h = nullptr;
+    int st = guarded([&] {
+        if (!params) ThrowException(InvalidArgument, "params", "cannot be null");
+        auto p = std::make_unique<Sched>();
+        p->inner = std::make_unique<MLPSchedule>(params, min_dim, max_dim, max_sub);
+        p->max_dim = max_dim;
+        h = p.release();
+    });
+    return st ? nullptr : h;
+}
+
+void rph_sched_free(void *h) { delete (Sched *)h; }
+
+int rph_sched_set_tracing(void *h, int on)
+{
+    if (!h) {
+        g_err = "null schedule";
+        return RP_ERR_INVALID_ARGUMENT;
+    }
+    ((Sched *)h)->tracing = on != 0;
+    ((Sched *)h)->trace.clear();
+    return 0;
+}
+
+// copies up to cap records: feats[cap][12]; meta[cap][4] = kind (0 sub-step
+// count, 1 Krylov dimension), value, restart, sub-step; upd[cap] = the
+// previous restart's update norm.  Returns the number of records in *count.
+int rph_sched_trace(void *h, float *feats, int *meta, double *upd, int cap, int *count)
+{
+    if (!h || !count) {
+        g_err = "null argument";
+        return RP_ERR_INVALID_ARGUMENT;
+    }
+    auto &tr = ((Sched *)h)->trace;
+    *count = (int)tr.size();
+    for (int i = 0; i < cap && i < (int)tr.size(); i++) {
+        if (feats) std::memcpy(feats + (size_t)i * RP_MLP_IN, tr[i].f, sizeof(tr[i].f));
+        if (meta) {
+            meta[4 * i] = tr[i].kind;
+            meta[4 * i + 1] = tr[i].value;
+            meta[4 * i + 2] = tr[i].restart;
+            meta[4 * i + 3] = tr[i].substep;
+        }
+        if (upd) upd[i] = tr[i].last_update;
+    }
+    return 0;
+}
+
+// evolve with a schedule handle; krydim is the fixed dimension a rejected
+// schedule falls back to
+int rph_state_evolve_sched(void *sp, void *hp, void *sched, double t, double beta, int krydim,
+                           double tol, int max_iters, int reorth, rph_stats *out)
+{
+    if (!sched) {
+        g_err = "null schedule";
+        return RP_ERR_INVALID_ARGUMENT;
+    }
+    KrylovOptions opt;
+    opt.krydim = krydim;
+    opt.tol = tol;
+    opt.max_iters = max_iters;
+    opt.reorth = reorth ? Reorth::Always : Reorth::Selective;
+    opt.schedule = (Sched *)sched;
+    opt.max_krydim = std::max(krydim, ((Sched *)sched)->max_dim);
+    return run_evolve(sp, hp, t, beta, opt, out, [](KrylovOptions &) {});
 }
 
 int rph_state_inner(void *ap, void *bp, double *re, double *im)

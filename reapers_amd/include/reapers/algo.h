@@ -55,13 +55,14 @@ class Krylov {
         return norm != 0 && norm >= eta * old_norm;
     }
 
-    // m-step Arnoldi from the normalized buf(0) (inc/algo.h:144-173)
+    // Arnoldi steps j0 .. m-1 from the orthonormal buf(0..j0) (inc/algo.h:144-173;
+    // j0 = 0 is the reference's m-step Arnoldi from the normalized buf(0))
     template<RealScalar FpType>
-    static bool arnoldi(State<FpType, Impl> &s, const SumOps<FpType> &A, dense::CMat &H, int &m,
-                        double &beta, Reorth mode, KrylovStats &st) {
+    static bool arnoldi(State<FpType, Impl> &s, const SumOps<FpType> &A, dense::CMat &H, int j0,
+                        int &m, double &beta, Reorth mode, KrylovStats &st) {
         const size_t dim = s.dim();
         const auto len = s.spin_chain_length();
-        for (int j = 0; j < m; j++) {
+        for (int j = j0; j < m; j++) {
             Impl::apply_ops_overwrite(len, s.buf(j + 1), A, s.buf(j));
             st.matvecs++;
             bool ok = orthogonalize(s, H, j, beta, mode, st);
@@ -75,28 +76,52 @@ class Krylov {
         return true;
     }
 
-    // one restarted evolution over the whole (t, beta)
+    // One restarted evolution over (t_real, t_imag).  Restart r uses Krylov
+    // dimension m_r: the options' krydim, or the schedule's proposal, made
+    // after the restart's first Arnoldi step (which every dimension shares).
+    // Hfull grows by blocks of those sizes.  A proposal is rejected -- and
+    // the options' krydim used for the rest of the evolution -- when a
+    // restart's update norm fails to shrink, the restarted method's
+    // a-posteriori sign that the block was too small for the remaining time.
     template<RealScalar FpType>
     static void evolve_once(State<FpType, Impl> &s, const SumOps<FpType> &ops, double t_real,
-                            double t_imag, const KrylovOptions &opt, KrylovStats &st) {
-        const int krydim = opt.krydim;
-        const double tol = opt.tol > 0 ? opt.tol : (double)epsilon<FpType>();
+                            double t_imag, const KrylovOptions &opt, RestartInfo &ri,
+                            KrylovStats &st) {
+        const double tol = ri.tol;
         const size_t dim = s.dim();
-        s.enlarge(krydim + 2);
-        Impl::zero_vec(dim, s.buf(krydim + 1));
+        int mmax = opt.krydim;
+        if (opt.schedule) mmax = std::max(mmax, opt.max_krydim);
+        s.enlarge(mmax + 2);
+        const int ans = mmax + 1;          // buffer index of the answer
+        Impl::zero_vec(dim, s.buf(ans));
         const double beta0 = Impl::vec_norm64(dim, s.buf());
         Impl::scale_vec(dim, s.buf(), (FpType)(1.0 / beta0));
         dense::CMat Hfull;
-        int niters = 0;
-        double oldbeta = 0.0;
+        int niters = 0, off = 0;
+        double oldbeta = 0.0, lastnrm = 0.0;
+        bool use_schedule = opt.schedule != nullptr;
+        ri.restart = 0;
+        ri.last_update = ri.prev_update = 0;
+        ri.prev_krydim = 0;
         while (true) {
             niters++;
-            int m = krydim;
+            dense::CMat H(mmax);
             double beta = 0;
-            dense::CMat H(krydim);
-            bool brokedown = !arnoldi(s, ops, H, m, beta, opt.reorth, st);
+            int m = 1;
+            bool brokedown = !arnoldi(s, ops, H, 0, m, beta, opt.reorth, st);
+            if (!brokedown) {
+                int krydim = opt.krydim;
+                if (use_schedule) {
+                    ri.alpha0 = H(0, 0).real();
+                    ri.beta0 = beta;
+                    krydim = std::max(2, std::min(opt.schedule->krydim(ri), mmax));
+                }
+                m = krydim;
+                H(1, 0) = beta;
+                brokedown = !arnoldi(s, ops, H, 1, m, beta, opt.reorth, st);
+            }
+            H.grow(m);
             auto t0 = std::chrono::steady_clock::now();
-            const int off = (niters - 1) * krydim;
             if (niters == 1) {
                 Hfull = H;
             } else {
@@ -120,16 +145,28 @@ class Krylov {
                               std::chrono::steady_clock::now() - t0).count();
             std::vector<const typename Impl::template VecType<FpType> *> V(m);
             for (int i = 0; i < m; i++) V[i] = &s.buf(i);
-            Impl::lincomb(dim, s.buf(krydim + 1), V, c);
+            Impl::lincomb(dim, s.buf(ans), V, c);
             const double nrm = std::sqrt(nrm2);
             st.last_nrm = nrm;
             if ((opt.max_iters && niters >= opt.max_iters) || brokedown || nrm < tol) {
                 st.breakdown |= brokedown ? 1 : 0;
-                s.inc_curbuf(krydim + 1);
+                // the answer becomes the current vector
+                s.swap_bufs(0, ans);
                 break;
             }
-            s.swap_bufs(0, krydim);
+            if (use_schedule && niters > 1 && nrm >= lastnrm) {
+                use_schedule = false;
+                st.rejected++;
+            }
+            lastnrm = nrm;
+            // restart from v_m
+            s.swap_bufs(0, m);
+            off += m;
             oldbeta = beta;
+            ri.restart = niters;
+            ri.prev_update = ri.last_update;
+            ri.last_update = nrm;
+            ri.prev_krydim = m;
         }
         st.restarts += niters;
     }
@@ -153,13 +190,37 @@ public:
     static void evolve_ex(State<FpType, Impl> &s, const SumOps<FpType> &ops, double t_real,
                           double t_imag, const KrylovOptions &opt, KrylovStats &st) {
         if (opt.krydim < 2) ThrowException(InvalidArgument, "krydim", "must be at least two");
+        RestartInfo ri;
+        ri.len = s.spin_chain_length();
+        ri.nterms = (long)ops.size();
+        double s2 = 0, s1 = 0;
+        for (auto &op : ops) {
+            double a = std::abs(std::complex<double>(op.get_coeff()));
+            s1 += a;
+            s2 += a * a;
+        }
+        ri.h_rms = std::sqrt(s2);
+        ri.h_l1 = s1;
+        if (opt.schedule) ri.apply_passes = Impl::apply_passes(ri.len, ops);
+        ri.t = t_real;
+        ri.beta = t_imag;
+        ri.tol = opt.tol > 0 ? opt.tol : (double)epsilon<FpType>();
         int nsub = 1;
-        if (opt.substep > 0) {
+        if (opt.schedule) {
+            ri.tau = t_real;
+            ri.taub = t_imag;
+            nsub = std::max(1, std::min(64, opt.schedule->substeps(ri)));
+        } else if (opt.substep > 0) {
             double span = std::max(std::abs(t_real), std::abs(t_imag));
             nsub = std::max(1, (int)std::ceil(span / opt.substep - 1e-12));
         }
-        for (int k = 0; k < nsub; k++)
-            evolve_once(s, ops, t_real / nsub, t_imag / nsub, opt, st);
+        ri.nsub = nsub;
+        ri.tau = t_real / nsub;
+        ri.taub = t_imag / nsub;
+        for (int k = 0; k < nsub; k++) {
+            ri.substep = k;
+            evolve_once(s, ops, t_real / nsub, t_imag / nsub, opt, ri, st);
+        }
         st.substeps += nsub;
     }
 

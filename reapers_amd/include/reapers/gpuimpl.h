@@ -246,6 +246,16 @@ public:
         RP_CALL(rp_apply_ops(ops.device_terms(len), res.ptr, vec.ptr, 0));
     }
 
+    // HBM traffic of one apply in units of one state vector (the uploaded
+    // layout's tile groups plus the output write)
+    template<RealScalar FpType>
+    static double apply_passes(typename SpinOp<FpType>::IndexType len,
+                               const DevSumOps<FpType> &ops) {
+        rp_terms_info_t info = ops.device_info(len);
+        double vec = (double)((size_t)1 << len) * 2 * sizeof(FpType);
+        return std::max(1.0, (double)info.bytes_per_apply / vec);
+    }
+
     // one fused Gram-Schmidt pass of w against V[0..k): h = V^H w (FP64),
     // w -= V h, returns ||w|| (FP64)
     template<RealScalar FpType>

@@ -68,10 +68,12 @@ template<RealScalar FpType = DefFpType>
 using SumOps = DefImpl::SumOps<FpType>;
 
 #include "state.h"
+#include "schedule.h"
 
 namespace Model {
 #include "fermions.h"
 #include "syk.h"
+#include "xxz.h"
 }
 
 }  // namespace REAPERS

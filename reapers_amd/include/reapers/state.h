@@ -18,11 +18,43 @@ struct KrylovStats {
     long gs_passes = 0;
     int breakdown = 0;
     int substeps = 0;
+    int rejected = 0;         // schedule proposals the error estimate rejected
     double last_nrm = 0;
     double host_ms = 0;       // time in the host-side small-matrix work
 };
 
 enum class Reorth { Selective, Always };
+
+// What a schedule sees when it decides the number of sub-steps (before the
+// evolution) and the Krylov dimension of a restart (after the restart's first
+// Arnoldi step, so that alpha0 and beta0 are this restart's).
+struct RestartInfo {
+    int len = 0;              // spin chain length
+    long nterms = 0;          // terms of H
+    double h_rms = 0;         // sqrt(sum |c|^2) = RMS eigenvalue of H
+    double h_l1 = 0;          // sum |c| >= spectral radius
+    double apply_passes = 1;  // HBM traffic of one H*v in state-vector sizes
+    double t = 0, beta = 0;   // the whole evolution
+    double tau = 0, taub = 0; // this sub-step
+    double tol = 0;
+    int substep = 0, nsub = 1;
+    int restart = 0;          // restarts done in this sub-step
+    double last_update = 0;   // ||Hexp0|| of the previous restart (0 before)
+    double prev_update = 0;   // ||Hexp0|| of the one before that (0 before)
+    int prev_krydim = 0;      // Krylov dimension of the previous restart
+    double alpha0 = 0;        // H(0,0) = <v0|H|v0> of this restart
+    double beta0 = 0;         // H(1,0) = ||H v0 - alpha0 v0|| of this restart
+};
+
+// Chooses the number of sub-steps of an evolution and the Krylov dimension
+// of every restart.  The default schedule is the reference's: one sub-step,
+// the same dimension for every restart.
+class KrylovSchedule {
+public:
+    virtual ~KrylovSchedule() {}
+    virtual int substeps(const RestartInfo &) { return 1; }
+    virtual int krydim(const RestartInfo &) = 0;
+};
 
 struct KrylovOptions {
     int krydim = 5;
@@ -30,6 +62,8 @@ struct KrylovOptions {
     int max_iters = 0;
     Reorth reorth = Reorth::Selective;
     double substep = 0;       // 0: whole t in one evolution
+    KrylovSchedule *schedule = nullptr;   // null: fixed krydim / substep
+    int max_krydim = 64;      // bound on a schedule's proposals
 };
 
 namespace EvolutionAlgorithm {

@@ -53,3 +53,50 @@ def test_fp32_bulk_vs_fp64(gpu, orc):
     err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
     print("fp32 rel err vs fp64 oracle %.3e matvecs %d" % (err, st.matvecs), flush=True)
     assert err < 1e-5
+
+
+@pytest.mark.parametrize("dims,nsub", [([4], 1), ([3, 5, 8], 2), ([8, 4], 3), ([2], 1)])
+def test_schedules_fp64(gpu, orc, dims, nsub):
+    """Per-restart Krylov dimensions and sub-steps converge to the same
+    exp(-iHt)psi (against ED)."""
+    N, L, t = 16, 8, 4.0
+    bits, coeff = orc.syk_terms(N, 0.0, seed=31, fp=64)
+    psi = random_state(np.random.default_rng(8), L)
+    ex = ed_evolve(orc.dense_matrix(L, bits, coeff), psi, t)
+    ham = gpu.Hamiltonian(L, bits, coeff, fp=64)
+    s = gpu.DeviceState(L, fp=64)
+    s.load(psi)
+    st = s.evolve_schedule(ham, t, dims, nsub=nsub, tol=1e-13)
+    got = s.store()
+    print("dims=%s nsub=%d matvecs=%d restarts=%d err=%.2e" % (
+        dims, nsub, st.matvecs, st.restarts, np.abs(got - ex).max()), flush=True)
+    assert np.abs(got - ex).max() < 1e-11
+    assert st.substeps == nsub
+
+
+def test_schedule_fp32_syk(gpu, orc):
+    N, L, t = 32, 16, 10.0
+    bits, coeff = orc.syk_terms(N, 8.0, seed=2000, fp=32)
+    psi = random_state(np.random.default_rng(9), L, np.complex64)
+    ref, _ = orc.krylov_evolve(L, bits, coeff, psi.astype(np.complex128), t, 0.0, krydim=30,
+                               tol=1e-14)
+    ham = gpu.Hamiltonian(L, bits, coeff, fp=32)
+    s = gpu.DeviceState(L, fp=32)
+    s.load(psi)
+    st = s.evolve_schedule(ham, t, [6], tol=1e-7)
+    err = np.linalg.norm(s.store().astype(np.complex128) - ref)
+    print("fp32 m=6: matvecs %d err %.2e" % (st.matvecs, err), flush=True)
+    assert err < 1e-6
+    assert st.matvecs <= 48
+
+
+def test_xxz_evolution(gpu, orc):
+    L, t = 10, 3.0
+    bits, coeff = gpu.xxz_terms(L, 1.0, 0.7, 3.0, seed=5, fp=64)
+    psi = random_state(np.random.default_rng(10), L)
+    ex = ed_evolve(orc.dense_matrix(L, bits, coeff), psi, t, 0.2)
+    ham = gpu.Hamiltonian(L, bits, coeff, fp=64)
+    s = gpu.DeviceState(L, fp=64)
+    s.load(psi)
+    s.evolve(ham, t, 0.2, krydim=12, tol=1e-13)
+    assert np.abs(s.store() - ex).max() < 1e-11
