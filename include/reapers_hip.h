@@ -167,6 +167,53 @@ int rp_prof_enable(int on);
 int rp_prof_reset(void);
 int rp_prof_query(int kernel_class, uint64_t *launches, double *total_ms);
 
+/* --- Batched right-hand sides (no counterpart in the reference, which
+ * evolves one State at a time, inc/state.h:413-424 and inc/algo.h:266-365;
+ * the per-problem arithmetic is the same as rp_apply_ops, rp_gs_pass and
+ * rp_lincomb).  A batch holds nprob Hamiltonians on len[p] qubits (12..30,
+ * uneven) with nrhs states each (1, 2 or 4).  A batch vector is one device
+ * allocation of info.vec_bytes bytes; problem p's block starts at basis
+ * offset off[p] (rp_batch_offsets) and state s of amplitude m is element
+ * (off[p] + m) * nrhs + s.  Every operation is one launch (or a fixed
+ * sequence of launches) over all problems whose active flag is set. */
+typedef struct rp_batch *rp_batch_t;
+typedef struct rp_graph *rp_graph_t;
+#define RP_BATCH_AUTOTUNE 1   /* time the instantiated tilings, keep the fastest */
+/* force the apply's tile bits b (9, 10 or 11; 11 only for 8-byte elements) */
+#define RP_BATCH_TILE_BITS(b) ((b) << 8)
+typedef struct {
+    int prec, nprob, nrhs, kmax;
+    int tile_bits, threads, grid;   /* apply tiling and persistent grid */
+    uint64_t nbasis;                /* sum of 2^len[p] */
+    uint64_t vec_bytes;             /* bytes of one batch vector */
+    uint64_t nterms, ngroups;       /* over all problems */
+    uint64_t bytes_per_apply;       /* sum_p (G_b,p + 1) 2^len[p] nrhs S */
+} rp_batch_info_t;
+/* terms of problem p: bits[term_off[p] .. term_off[p+1]), coeff interleaved;
+ * kmax bounds the Krylov dimension of rp_batch_arnoldi_step */
+int rp_batch_create(rp_batch_t *out, int prec, int nprob, const int *len, const uint64_t *bits,
+                    const double *coeff, const int64_t *term_off, int nrhs, int kmax, int flags);
+int rp_batch_free(rp_batch_t b);
+int rp_batch_info(rp_batch_t b, rp_batch_info_t *info);
+int rp_batch_offsets(rp_batch_t b, uint64_t *off);              /* nprob + 1 entries */
+int rp_batch_set_active(rp_batch_t b, const int *active_host);  /* nprob flags */
+/* the basis table V_0..V_{n-1} read by the Arnoldi step and lincomb */
+int rp_batch_set_basis(rp_batch_t b, void *const *V, int n);
+/* res (+)= H_p vec_p for every active problem */
+int rp_batch_apply(rp_batch_t b, void *res, const void *vec, int accumulate);
+/* V_i /= ||V_i|| per state; nrm_dev[p * nrhs + s] = the norm (device) */
+int rp_batch_normalize(rp_batch_t b, int i, double *nrm_dev);
+/* Arnoldi step j: V_{j+1} = A V_j, two Gram-Schmidt passes (FP64 dots),
+ * H(0..j+1, j) into H_dev (per state a column-major (kmax+1) x kmax complex
+ * double matrix, states consecutive), V_{j+1} normalized; kernels only */
+int rp_batch_arnoldi_step(rp_batch_t b, int j, double *H_dev);
+/* y += sum_{i<k} c[rhs][i] V_i, c complex double (device), cstride doubles per state */
+int rp_batch_lincomb(rp_batch_t b, void *y, int k, const double *c_dev, int cstride);
+/* Arnoldi steps j0..j1-1 captured as one HIP graph (no memset or copy nodes) */
+int rp_batch_capture_arnoldi(rp_batch_t b, int j0, int j1, double *H_dev, rp_graph_t *out);
+int rp_graph_launch(rp_graph_t g);
+int rp_graph_free(rp_graph_t g);
+
 /* --- Krylov-schedule proposer network (no counterpart in the reference,
  * whose Krylov::evolve uses one fixed dimension for all restarts,
  * inc/algo.h:266-269).  A 12 -> 64 -> 64 -> 2 FP32 perceptron with tanh

@@ -52,6 +52,13 @@ class EvolveStats(_c.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class BatchInfo(_c.Structure):
+    _fields_ = [("prec", _c.c_int), ("nprob", _c.c_int), ("nrhs", _c.c_int), ("kmax", _c.c_int),
+                ("tile_bits", _c.c_int), ("threads", _c.c_int), ("grid", _c.c_int),
+                ("nbasis", _u64), ("vec_bytes", _u64), ("nterms", _u64), ("ngroups", _u64),
+                ("bytes_per_apply", _u64)]
+
+
 _hip = None
 _host = None
 
@@ -141,6 +148,17 @@ def host():
             "rph_sched_trace": ([_vp, _vp, _vp, _vp, _c.c_int, _c.POINTER(_c.c_int)], _c.c_int),
             "rph_state_evolve_sched": ([_vp, _vp, _vp, _dbl, _dbl, _c.c_int, _dbl, _c.c_int,
                                         _c.c_int, _c.POINTER(EvolveStats)], _c.c_int),
+            "rph_otoc": ([_vp, _vp, _vp, _vp, _dbl, _vp, _c.c_int, _c.c_int, _dbl, _vp,
+                          _c.POINTER(EvolveStats)], _c.c_int),
+            "rph_batch_new": ([_c.c_int, _c.c_int, _vp, _vp, _vp, _vp, _c.c_int, _c.c_int,
+                               _c.c_int], _vp),
+            "rph_batch_free": ([_vp], None),
+            "rph_batch_info": ([_vp, _c.POINTER(BatchInfo), _vp], _c.c_int),
+            "rph_batch_load": ([_vp, _vp], _c.c_int),
+            "rph_batch_store": ([_vp, _vp], _c.c_int),
+            "rph_batch_apply": ([_vp], _c.c_int),
+            "rph_batch_evolve": ([_vp, _dbl, _dbl, _c.c_int, _dbl, _c.c_int, _c.c_int,
+                                  _c.POINTER(EvolveStats), _vp], _c.c_int),
             "rph_state_inner": ([_vp, _vp, _c.POINTER(_dbl), _c.POINTER(_dbl)], _c.c_int),
             "rph_state_norm": ([_vp, _c.POINTER(_dbl)], _c.c_int),
             "rph_state_ground": ([_vp, _vp, _c.c_int, _dbl, _c.c_int, _c.POINTER(_dbl)], _c.c_int),
@@ -324,6 +342,16 @@ class DeviceState:
                                              max_iters, reorth, _c.byref(st)))
         return st
 
+    def otoc(self, ham, A, B, times, beta=0.0, krydim=12, tol=0.0):
+        """F(t) = <i|A(t) B A(t) B|i> for each t, |i> this state (which
+        becomes exp(-beta H/8)|psi> when beta != 0).  Returns (F, stats)."""
+        ts = np.ascontiguousarray(times, dtype=np.float64)
+        out = np.zeros(2 * len(ts))
+        st = EvolveStats()
+        hcheck(host().rph_otoc(ham._h, A._h, B._h, self._s, beta, _ptr(ts), len(ts), krydim,
+                               tol, _ptr(out), _c.byref(st)))
+        return out[0::2] + 1j * out[1::2], st
+
     def inner(self, other):
         re, im = _dbl(), _dbl()
         hcheck(host().rph_state_inner(self._s, other._s, _c.byref(re), _c.byref(im)))
@@ -468,3 +496,78 @@ def mlp_infer(params, x):
     y = np.empty((x.shape[0], MLP_OUT), np.float32)
     check(hip().rp_mlp_infer(params.ptr, _ptr(x), x.shape[0], _ptr(y)))
     return y
+
+
+# ---------------------------------------------------------------------------
+# batches of right-hand sides
+# ---------------------------------------------------------------------------
+class Batch:
+    """BatchedStates<Fp>: problems (lens[p], bits[p], coeff[p]) with nrhs
+    states each in the block-state layout."""
+
+    def __init__(self, lens, terms, nrhs=4, kmax=20, fp=32, autotune=True, tile_bits=0):
+        self.lens = [int(x) for x in lens]
+        self.nprob = len(self.lens)
+        self.nrhs = nrhs
+        self.fp = fp
+        bits = np.ascontiguousarray(np.concatenate([np.asarray(b, np.uint64) for b, _ in terms]))
+        coeff = np.concatenate([np.asarray(c, np.complex128) for _, c in terms])
+        co = np.ascontiguousarray(np.stack([coeff.real, coeff.imag], 1).ravel())
+        toff = np.zeros(self.nprob + 1, np.int64)
+        toff[1:] = np.cumsum([len(b) for b, _ in terms])
+        la = np.array(self.lens, np.int32)
+        self._h = host().rph_batch_new(fp, self.nprob, _ptr(la), _ptr(bits), _ptr(co), _ptr(toff),
+                                       nrhs, kmax, (tile_bits << 8) if tile_bits
+                                       else (1 if autotune else 0))
+        if not self._h:
+            raise RpError(-1, host().rph_last_error().decode())
+        self.info = BatchInfo()
+        self.offsets = np.zeros(self.nprob + 1, np.uint64)
+        hcheck(host().rph_batch_info(self._h, _c.byref(self.info), _ptr(self.offsets)))
+
+    @property
+    def dtype(self):
+        return np.complex64 if self.fp == 32 else np.complex128
+
+    def pack(self, states):
+        """states[p]: array (2^lens[p], nrhs) -> the batch vector (host)."""
+        out = np.zeros((int(self.info.nbasis), self.nrhs), self.dtype)
+        for p, x in enumerate(states):
+            o = int(self.offsets[p])
+            out[o:o + (1 << self.lens[p])] = x
+        return out.ravel()
+
+    def unpack(self, v):
+        v = v.reshape(-1, self.nrhs)
+        return [v[int(self.offsets[p]):int(self.offsets[p]) + (1 << self.lens[p])].copy()
+                for p in range(self.nprob)]
+
+    def load(self, states):
+        a = np.ascontiguousarray(self.pack(states))
+        hcheck(host().rph_batch_load(self._h, _ptr(a)))
+
+    def store(self):
+        a = np.empty(int(self.info.nbasis) * self.nrhs, self.dtype)
+        hcheck(host().rph_batch_store(self._h, _ptr(a)))
+        return self.unpack(a)
+
+    def apply(self):
+        hcheck(host().rph_batch_apply(self._h))
+
+    def evolve(self, t, m=12, beta=0.0, tol=0.0, max_iters=0, graph=True):
+        st = EvolveStats()
+        rs = np.zeros(self.nprob * self.nrhs, np.int32)
+        hcheck(host().rph_batch_evolve(self._h, t, beta, m, tol, max_iters, 1 if graph else 0,
+                                       _c.byref(st), _ptr(rs)))
+        return st, rs.reshape(self.nprob, self.nrhs)
+
+    def close(self):
+        if self._h:
+            host().rph_batch_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -16,7 +16,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 
-HIP_SOURCES = ["rp_runtime.hip", "rp_vec.hip", "rp_apply.hip", "rp_mlp.hip"]
+HIP_SOURCES = ["rp_runtime.hip", "rp_vec.hip", "rp_apply.hip", "rp_mlp.hip", "rp_batch.hip"]
 HOST_SOURCES = ["host_api.cc"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

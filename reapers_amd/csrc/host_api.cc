@@ -419,6 +419,138 @@ int rph_state_evolve_list(void *sp, void *hp, double t, double beta, const int *
     return run_evolve(sp, hp, t, beta, opt, out, [](KrylovOptions &) {});
 }
 
+// OTOC F(t_k) = <i|A(t)BA(t)B|i> of the state handle sp (which becomes
+// exp(-beta H/8)|psi> if beta != 0); out[2k], out[2k+1] = Re, Im F(t_k)
+int rph_otoc(void *hp, void *ap, void *bp, void *sp, double beta, const double *times,
+             int ntimes, int krydim, double tol, double *out, rph_stats *stats)
+{
+    return guarded([&] {
+        Ham *h = (Ham *)hp, *a = (Ham *)ap, *b = (Ham *)bp;
+        St *s = (St *)sp;
+        if (h->fp != s->fp || a->fp != s->fp || b->fp != s->fp)
+            ThrowException(InvalidArgument, "precisions of state and operators must match");
+        if (ntimes < 0 || (ntimes && (!times || !out)))
+            ThrowException(InvalidArgument, "times", "null or negative count");
+        KrylovOptions opt;
+        opt.krydim = krydim;
+        opt.tol = tol;
+        std::vector<double> ts(times, times + ntimes);
+        KrylovStats st;
+        auto t0 = std::chrono::steady_clock::now();
+        auto v = s->fp == 32 ? otoc<float, GPUImpl>(*h->f, *a->f, *b->f, *s->f, ts, beta, opt, st)
+                             : otoc<double, GPUImpl>(*h->d, *a->d, *b->d, *s->d, ts, beta, opt, st);
+        GPUImpl::synchronize();
+        double wall = std::chrono::duration<double, std::milli>(
+                          std::chrono::steady_clock::now() - t0).count();
+        for (int k = 0; k < ntimes; k++) {
+            out[2 * k] = v[k].real();
+            out[2 * k + 1] = v[k].imag();
+        }
+        fill_stats(stats, st, wall);
+    });
+}
+
+// batches of right-hand sides ----------------------------------------------
+
+struct Batch {
+    int fp;
+    BatchedStates<float> *f = nullptr;
+    BatchedStates<double> *d = nullptr;
+    ~Batch() { delete f; delete d; }
+};
+
+// nprob problems: chain length lens[p], terms bits/coeff[toff[p] .. toff[p+1])
+void *rph_batch_new(int fp, int nprob, const int *lens, const uint64_t *bits, const double *coeff,
+                    const int64_t *toff, int nrhs, int kmax, int flags)
+{
+    Batch *h = nullptr;
+    int st = guarded([&] {
+        if (fp != 32 && fp != 64) ThrowException(InvalidArgument, "fp", "must be 32 or 64");
+        if (nprob < 1 || !lens || !toff) ThrowException(InvalidArgument, "problems", "none given");
+        auto p = std::make_unique<Batch>();
+        p->fp = fp;
+        std::vector<int> L(lens, lens + nprob);
+        if (fp == 32) {
+            std::vector<HostSumOps<float>> hs;
+            for (int q = 0; q < nprob; q++)
+                hs.push_back(HostSumOps<float>::from_terms(bits + toff[q], coeff + 2 * toff[q],
+                                                           (size_t)(toff[q + 1] - toff[q])));
+            std::vector<const HostSumOps<float> *> ptr;
+            for (auto &x : hs) ptr.push_back(&x);
+            p->f = new BatchedStates<float>(ptr, L, nrhs, kmax, flags);
+        } else {
+            std::vector<HostSumOps<double>> hs;
+            for (int q = 0; q < nprob; q++)
+                hs.push_back(HostSumOps<double>::from_terms(bits + toff[q], coeff + 2 * toff[q],
+                                                            (size_t)(toff[q + 1] - toff[q])));
+            std::vector<const HostSumOps<double> *> ptr;
+            for (auto &x : hs) ptr.push_back(&x);
+            p->d = new BatchedStates<double>(ptr, L, nrhs, kmax, flags);
+        }
+        h = p.release();
+    });
+    return st ? nullptr : h;
+}
+
+void rph_batch_free(void *h) { delete (Batch *)h; }
+
+int rph_batch_info(void *h, rp_batch_info_t *info, uint64_t *offsets)
+{
+    return guarded([&] {
+        Batch *b = (Batch *)h;
+        const rp_batch_info_t &i = b->fp == 32 ? b->f->layout() : b->d->layout();
+        if (info) *info = i;
+        if (offsets) {
+            const auto &o = b->fp == 32 ? b->f->offsets() : b->d->offsets();
+            std::copy(o.begin(), o.end(), offsets);
+        }
+    });
+}
+
+int rph_batch_load(void *h, const void *host)
+{
+    return guarded([&] {
+        Batch *b = (Batch *)h;
+        if (b->fp == 32) b->f->load((const complex<float> *)host);
+        else b->d->load((const complex<double> *)host);
+    });
+}
+
+int rph_batch_store(void *h, void *host)
+{
+    return guarded([&] {
+        Batch *b = (Batch *)h;
+        if (b->fp == 32) b->f->store((complex<float> *)host);
+        else b->d->store((complex<double> *)host);
+    });
+}
+
+int rph_batch_apply(void *h)
+{
+    return guarded([&] {
+        Batch *b = (Batch *)h;
+        if (b->fp == 32) b->f->apply();
+        else b->d->apply();
+        GPUImpl::synchronize();
+    });
+}
+
+int rph_batch_evolve(void *h, double t, double beta, int m, double tol, int max_iters,
+                     int use_graph, rph_stats *out, int *restarts)
+{
+    return guarded([&] {
+        Batch *b = (Batch *)h;
+        std::vector<int> rs;
+        auto t0 = std::chrono::steady_clock::now();
+        KrylovStats st = b->fp == 32 ? b->f->evolve(t, beta, m, tol, max_iters, use_graph != 0, &rs)
+                                     : b->d->evolve(t, beta, m, tol, max_iters, use_graph != 0, &rs);
+        double wall = std::chrono::duration<double, std::milli>(
+                          std::chrono::steady_clock::now() - t0).count();
+        fill_stats(out, st, wall);
+        if (restarts) std::copy(rs.begin(), rs.end(), restarts);
+    });
+}
+
 // schedule handles --------------------------------------------------------
 
 void *rph_sched_list_new(const int *dims, int ndims, int nsub)

@@ -33,26 +33,13 @@
 // output tiles, whose input tiles overlap in that XCD's L2.
 // HBM traffic per apply: (G_b + 1) * 2^len * S bytes, G_b = number of groups.
 #include "rp_internal.h"
+#include "rp_tables.h"
 
 #include <algorithm>
 #include <cstring>
 #include <vector>
 
 namespace rp {
-
-template <typename R> struct CxA;
-template <> struct CxA<float> { using T = float2; };
-template <> struct CxA<double> { using T = double2; };
-
-// kinds of subs: one term with real / imaginary / complex c, or several terms
-enum : uint32_t { kSubReal = 0, kSubImag = 1, kSubComplex = 2, kSubMulti = 3 };
-
-template <typename A> struct SubRec {
-    uint32_t xl, z, kind, term_begin, term_end, pad0, pad1, pad2;
-    A cr, ci;
-};
-template <typename A> struct TermRec { uint32_t z, pad; A cr, ci; };
-template <typename A> struct FlatTerm { uint32_t x, z; A cr, ci; };
 
 constexpr int kSmallThreads = 256;
 
@@ -69,8 +56,6 @@ struct rp_terms {
 };
 
 namespace rp {
-
-__device__ inline uint32_t parity32(uint32_t v) { return __popc(v) & 1u; }
 
 // one thread per amplitude; terms in (x-sorted) flat order
 template <typename R, typename A>
@@ -101,13 +86,6 @@ k_apply_small(const typename CxA<R>::T *__restrict__ in, typename CxA<R>::T *__r
     o.y = (R)ai;
     out[m] = o;
 }
-
-// Sub classes, in the order a group stores its subs: single-term real c with
-// no phase bit in the row bits, real with row bits, imaginary (same split),
-// complex, and several terms per flip mask.
-enum : uint32_t { kRunReal0 = 0, kRunReal1, kRunImag0, kRunImag1, kRunCplx, kRunMulti, kRuns };
-
-struct GroupRuns { uint32_t xh, b[kRuns + 1]; };
 
 template <typename R, int TB, int NT>
 constexpr int apply_waves_per_eu()
@@ -313,20 +291,6 @@ k_apply_tiled(const typename CxA<R>::T *__restrict__ in, typename CxA<R>::T *__r
     }
 }
 
-static inline uint32_t pext_even(uint64_t b)   // bit 2s -> bit s
-{
-    uint32_t r = 0;
-    for (int s = 0; s < 32; s++) r |= (uint32_t)((b >> (2 * s)) & 1u) << s;
-    return r;
-}
-
-static inline uint32_t pext_odd(uint64_t b)    // bit 2s+1 -> bit s
-{
-    uint32_t r = 0;
-    for (int s = 0; s < 32; s++) r |= (uint32_t)((b >> (2 * s + 1)) & 1u) << s;
-    return r;
-}
-
 // tile bits b and workgroup size: 8 amplitudes per thread, two 64 KiB LDS
 // tiles (2^13 complex<float> or 2^12 complex<double>) at full size, smaller
 // tiles and workgroups for short chains so that there are enough workgroups
@@ -343,86 +307,16 @@ static void choose_tiling(int prec, int len, int *tb, int *nt)
 template <typename R, typename A>
 static int upload_impl(rp_terms *h, const uint64_t *bits, const double *coeff, uint64_t n)
 {
-    struct Tmp { uint32_t x, z; A cr, ci; };
-    std::vector<Tmp> v;
-    v.reserve(n);
-    const uint64_t sitemask = h->len == 32 ? ~0ull : ((1ull << (2 * h->len)) - 1);
-    for (uint64_t t = 0; t < n; t++) {
-        // coefficients arrive as doubles holding values of the SumOps' FpType
-        A cr = (A)(R)coeff[2 * t], ci = (A)(R)coeff[2 * t + 1];
-        if (cr == 0 && ci == 0) continue;
-        if (bits[t] & ~sitemask)
-            return set_error(RP_ERR_INVALID_ARGUMENT,
-                             "rp_terms_upload: term %llu acts on a site >= len=%d",
-                             (unsigned long long)t, h->len);
-        Tmp e;
-        e.x = pext_even(bits[t]);
-        e.z = pext_odd(bits[t]);
-        if (__builtin_popcount(e.x & e.z) & 1) { cr = -cr; ci = -ci; }
-        e.cr = cr;
-        e.ci = ci;
-        v.push_back(e);
-    }
+    Tables<A> tab;
+    int st = build_tables<R, A>(h->len, h->tile_bits, h->threads, bits, coeff, n, tab);
+    if (st) return st;
+    auto &groups = tab.groups;
+    auto &subs = tab.subs;
+    auto &terms = tab.terms;
+    auto &flat = tab.flat;
     const int tb = h->tile_bits;
-    const uint32_t lowmask = tb ? ((1u << tb) - 1) : 0xffffffffu;
-    std::stable_sort(v.begin(), v.end(), [&](const Tmp &a, const Tmp &b) {
-        uint32_t ah = tb ? a.x >> tb : 0, bh = tb ? b.x >> tb : 0;
-        if (ah != bh) return ah < bh;
-        return (a.x & lowmask) < (b.x & lowmask);
-    });
-    h->nterms = v.size();
-    std::vector<GroupRuns> groups;
-    std::vector<SubRec<A>> subs;
-    std::vector<TermRec<A>> terms;
-    std::vector<FlatTerm<A>> flat;
-    uint64_t nflips = 0;
-    for (size_t i = 0; i < v.size(); i++) {
-        if (i == 0 || v[i].x != v[i - 1].x) nflips++;
-        flat.push_back({v[i].x, v[i].z, v[i].cr, v[i].ci});
-    }
-    if (tb) {
-        const int lb = __builtin_ctz((unsigned)h->threads);
-        const uint32_t rpmask = (1u << (tb - lb)) - 1;
-        size_t i = 0;
-        while (i < v.size()) {
-            const uint32_t xh = v[i].x >> tb;
-            // the group's subs, classified, then stored run by run
-            std::vector<SubRec<A>> run[kRuns];
-            while (i < v.size() && (v[i].x >> tb) == xh) {
-                size_t j = i;
-                while (j < v.size() && v[j].x == v[i].x) j++;
-                SubRec<A> S;
-                memset(&S, 0, sizeof(S));
-                S.xl = v[i].x & lowmask;
-                int cls;
-                if (j - i == 1) {
-                    S.z = v[i].z;
-                    S.cr = v[i].cr;
-                    S.ci = v[i].ci;
-                    const bool rowz = ((S.z >> lb) & rpmask) != 0;
-                    if (v[i].ci == 0) { S.kind = kSubReal; cls = rowz ? kRunReal1 : kRunReal0; }
-                    else if (v[i].cr == 0) { S.kind = kSubImag; cls = rowz ? kRunImag1 : kRunImag0; }
-                    else { S.kind = kSubComplex; cls = kRunCplx; }
-                } else {
-                    S.kind = kSubMulti;
-                    S.term_begin = (uint32_t)terms.size();
-                    for (size_t q = i; q < j; q++) terms.push_back({v[q].z, 0, v[q].cr, v[q].ci});
-                    S.term_end = (uint32_t)terms.size();
-                    cls = kRunMulti;
-                }
-                run[cls].push_back(S);
-                i = j;
-            }
-            GroupRuns G;
-            G.xh = xh;
-            for (int k = 0; k < kRuns; k++) {
-                G.b[k] = (uint32_t)subs.size();
-                subs.insert(subs.end(), run[k].begin(), run[k].end());
-            }
-            G.b[kRuns] = (uint32_t)subs.size();
-            groups.push_back(G);
-        }
-    }
+    const uint64_t nflips = tab.nflips;
+    h->nterms = tab.nterms;
     h->nflips = nflips;
     h->ngroups = tb ? groups.size() : 1;
     h->nsubs = subs.size();
@@ -438,7 +332,6 @@ static int upload_impl(rp_terms *h, const uint64_t *bits, const double *coeff, u
         RP_CHECK_HIP(hipMemcpyAsync(*dst, src, bytes, hipMemcpyHostToDevice, c.stream));
         return RP_OK;
     };
-    int st;
     if ((st = up(&h->d_flat, flat.data(), flat.size() * sizeof(FlatTerm<A>)))) return st;
     if (tb) {
         if ((st = up(&h->d_groups, groups.data(), groups.size() * sizeof(GroupRuns)))) return st;

@@ -1,0 +1,215 @@
+// batch.h -- many right-hand sides evolved together: nprob Hamiltonians of
+// uneven size (disorder realizations), nrhs states each, in the block-state
+// layout of libreapers_hip.so's batch entry points.  No counterpart in the
+// reference, which evolves one State at a time (inc/state.h:413-424); per
+// right-hand side the arithmetic is the reference's restarted Krylov
+// (inc/algo.h:266-365): Arnoldi from the normalized state, the Hessenberg
+// block appended to Hfull with the previous restart's beta at
+// (off, off - 1), Hexp = exp(-(beta + i t) Hfull), the answer updated with
+// beta0 Hexp[off + i, 0] V_i, convergence when that column block's 2-norm
+// drops below tol, restart from V_m.
+//
+// Differences from the single-state evolution: every Arnoldi step makes two
+// Gram-Schmidt passes (a fixed kernel sequence, so that a restart's m steps
+// are one captured HIP graph replayed for every restart); the small-matrix
+// exponentials of all right-hand sides run on the host in parallel between
+// restarts; a right-hand side that has converged stops contributing, and a
+// problem whose states have all converged is skipped by every kernel.
+#pragma once
+
+#ifndef _REAPERS_H_
+#error "Do not include this file directly. Include the master header file reapers.h"
+#endif
+
+template<RealScalar FpType>
+class BatchedStates {
+    rp_batch_t b = nullptr;
+    rp_batch_info_t info{};
+    std::vector<uint64_t> off;
+    std::vector<int> lens;
+    int nprob = 0, ns = 1, kmax = 0;
+    std::vector<void *> bufs;   // bufs[0]: the states; then Krylov vectors, answer
+    double *H_dev = nullptr, *nrm_dev = nullptr, *c_dev = nullptr;
+
+    static constexpr int prec = std::is_same_v<FpType, float> ? RP_C64 : RP_C128;
+
+    void *alloc_vec() {
+        void *p = nullptr;
+        RP_CALL(rp_malloc(&p, info.vec_bytes));
+        return p;
+    }
+    void enlarge(int n) {
+        while ((int)bufs.size() < n) bufs.push_back(alloc_vec());
+    }
+
+public:
+    // hams[p] acts on lens[p] qubits; kmax bounds the Krylov dimension; flags
+    // as rp_batch_create (RP_BATCH_AUTOTUNE, RP_BATCH_TILE_BITS(b))
+    BatchedStates(const std::vector<const HostSumOps<FpType> *> &hams, std::vector<int> lens_,
+                  int nrhs, int kmax_, int flags = RP_BATCH_AUTOTUNE)
+        : lens(std::move(lens_)), nprob((int)hams.size()), ns(nrhs), kmax(kmax_) {
+        if ((int)lens.size() != nprob)
+            ThrowException(InvalidArgument, "lens", "must have one entry per Hamiltonian");
+        std::vector<u64> bits;
+        std::vector<double> coeff;
+        std::vector<int64_t> toff(1, 0);
+        for (auto *h : hams) {
+            std::vector<u64> b1;
+            std::vector<double> c1;
+            h->export_terms(b1, c1);
+            bits.insert(bits.end(), b1.begin(), b1.end());
+            coeff.insert(coeff.end(), c1.begin(), c1.end());
+            toff.push_back((int64_t)bits.size());
+        }
+        RP_CALL(rp_batch_create(&b, prec, nprob, lens.data(), bits.data(), coeff.data(),
+                                toff.data(), nrhs, kmax, flags));
+        RP_CALL(rp_batch_info(b, &info));
+        off.resize(nprob + 1);
+        RP_CALL(rp_batch_offsets(b, off.data()));
+        const size_t nr = (size_t)nprob * ns;
+        void *p;
+        RP_CALL(rp_malloc(&p, nr * (kmax + 1) * kmax * 2 * sizeof(double)));
+        H_dev = (double *)p;
+        RP_CALL(rp_malloc(&p, nr * sizeof(double)));
+        nrm_dev = (double *)p;
+        RP_CALL(rp_malloc(&p, nr * 2 * kmax * sizeof(double)));
+        c_dev = (double *)p;
+        bufs.push_back(alloc_vec());
+        RP_CALL(rp_zero_vec(prec, bufs[0], info.nbasis * ns));
+    }
+    BatchedStates(const BatchedStates &) = delete;
+    BatchedStates &operator=(const BatchedStates &) = delete;
+    ~BatchedStates() {
+        for (void *p : bufs) rp_free(p);
+        for (void *p : {(void *)H_dev, (void *)nrm_dev, (void *)c_dev})
+            if (p) rp_free(p);
+        if (b) rp_batch_free(b);
+    }
+
+    const rp_batch_info_t &layout() const { return info; }
+    const std::vector<uint64_t> &offsets() const { return off; }
+    size_t size() const { return info.nbasis * ns; }   // complex elements
+
+    void load(const complex<FpType> *host) { RP_CALL(rp_memcpy(bufs[0], host, info.vec_bytes, RP_H2D)); }
+    void store(complex<FpType> *host) const { RP_CALL(rp_memcpy(host, bufs[0], info.vec_bytes, RP_D2H)); }
+
+    // states = H states (every problem)
+    void apply() {
+        enlarge(2);
+        std::vector<int> act(nprob, 1);
+        RP_CALL(rp_batch_set_active(b, act.data()));
+        RP_CALL(rp_batch_apply(b, bufs[1], bufs[0], 0));
+        std::swap(bufs[0], bufs[1]);
+    }
+
+    // states = exp(-(beta + i t) H_p) states, restarted Krylov of dimension m
+    // per right-hand side; restarts[rhs] (optional) receives the restarts each
+    // right-hand side needed
+    KrylovStats evolve(double t, double beta, int m, double tol, int max_iters = 0,
+                       bool use_graph = true, std::vector<int> *restarts = nullptr) {
+        using cd = std::complex<double>;
+        if (m < 2 || m > kmax) ThrowException(InvalidArgument, "m", "must be in [2, kmax]");
+        KrylovStats st;
+        const int nr = nprob * ns;
+        // buffers: V_0 .. V_m at bufs[1 .. m+1], the answer at bufs[m+2]
+        enlarge(m + 3);
+        std::vector<void *> V(bufs.begin() + 1, bufs.begin() + m + 2);
+        void *ans = bufs[m + 2];
+        RP_CALL(rp_memcpy(V[0], bufs[0], info.vec_bytes, RP_D2D));
+        RP_CALL(rp_batch_set_basis(b, V.data(), m + 1));
+        std::vector<int> act(nprob, 1);
+        RP_CALL(rp_batch_set_active(b, act.data()));
+        RP_CALL(rp_batch_normalize(b, 0, nrm_dev));
+        std::vector<double> beta0(nr);
+        RP_CALL(rp_memcpy(beta0.data(), nrm_dev, nr * sizeof(double), RP_D2H));
+        RP_CALL(rp_zero_vec(prec, ans, info.nbasis * ns));
+        std::vector<char> done(nr, 0);
+        std::vector<int> nrest(nr, 0);
+        for (int r = 0; r < nr; r++) done[r] = beta0[r] == 0;
+        rp_graph_t graph = nullptr;
+        if (use_graph) RP_CALL(rp_batch_capture_arnoldi(b, 0, m, H_dev, &graph));
+        std::vector<dense::CMat> Hfull(nr);
+        std::vector<double> oldbeta(nr, 0.0);
+        const size_t hstride = (size_t)2 * (kmax + 1) * kmax;
+        std::vector<double> Hh(hstride * nr), ch((size_t)2 * kmax * nr);
+        int niters = 0, off_blk = 0;
+        const cd z{-beta, -t};
+        try {
+            while (true) {
+                niters++;
+                for (int p = 0; p < nprob; p++) {
+                    act[p] = 0;
+                    for (int s = 0; s < ns; s++) act[p] |= !done[p * ns + s];
+                }
+                RP_CALL(rp_batch_set_active(b, act.data()));
+                if (graph) {
+                    RP_CALL(rp_graph_launch(graph));
+                } else {
+                    for (int j = 0; j < m; j++) RP_CALL(rp_batch_arnoldi_step(b, j, H_dev));
+                }
+                for (int p = 0; p < nprob; p++)
+                    if (act[p]) st.matvecs += (long)m * ns;
+                RP_CALL(rp_memcpy(Hh.data(), H_dev, Hh.size() * sizeof(double), RP_D2H));
+                auto t0 = std::chrono::steady_clock::now();
+                std::fill(ch.begin(), ch.end(), 0.0);
+                std::vector<char> now_done(done);
+#pragma omp parallel for schedule(dynamic)
+                for (int r = 0; r < nr; r++) {
+                    if (done[r]) continue;
+                    const double *Hr = Hh.data() + hstride * r;
+                    auto Hm = [&](int i, int j) {
+                        const double *q = Hr + 2 * ((size_t)j * (kmax + 1) + i);
+                        return cd(q[0], q[1]);
+                    };
+                    dense::CMat &F = Hfull[r];
+                    F.grow(off_blk + m);
+                    if (off_blk) F(off_blk, off_blk - 1) = oldbeta[r];
+                    for (int i = 0; i < m; i++)
+                        for (int j = 0; j < m; j++) F(off_blk + i, off_blk + j) = i <= j + 1 ? Hm(i, j) : 0.0;
+                    dense::CMat tH = F;
+                    for (auto &v : tH.a) v *= z;
+                    dense::CMat E = dense::expm(tH);
+                    double nrm2 = 0;
+                    for (int i = 0; i < m; i++) {
+                        cd e = E(off_blk + i, 0);
+                        cd c = beta0[r] * e;
+                        ch[(size_t)2 * kmax * r + 2 * i] = c.real();
+                        ch[(size_t)2 * kmax * r + 2 * i + 1] = c.imag();
+                        nrm2 += std::norm(e);
+                    }
+                    oldbeta[r] = Hm(m, m - 1).real();
+                    nrest[r] = niters;
+                    // a zero norm (inverse 0 on the device) is the Arnoldi
+                    // breakdown: the Krylov space is invariant and the
+                    // answer exact (inc/algo.h:356)
+                    bool brk = false;
+                    for (int j = 0; j < m; j++) brk = brk || Hm(j + 1, j) == 0.0;
+                    if (brk || std::sqrt(nrm2) < tol || (max_iters && niters >= max_iters))
+                        now_done[r] = 1;
+                }
+                st.host_ms += std::chrono::duration<double, std::milli>(
+                                  std::chrono::steady_clock::now() - t0).count();
+                RP_CALL(rp_memcpy(c_dev, ch.data(), ch.size() * sizeof(double), RP_H2D));
+                RP_CALL(rp_batch_lincomb(b, ans, m, c_dev, 2 * kmax));
+                done = now_done;
+                off_blk += m;
+                bool all = true;
+                for (char d : done) all = all && d;
+                if (all) break;
+                // restart from V_m: the graph's pointers stay fixed, so copy
+                RP_CALL(rp_memcpy(V[0], V[m], info.vec_bytes, RP_D2D));
+            }
+        } catch (...) {
+            if (graph) rp_graph_free(graph);
+            throw;
+        }
+        if (graph) RP_CALL(rp_graph_free(graph));
+        std::swap(bufs[0], bufs[m + 2]);
+        RP_CALL(rp_synchronize());
+        st.restarts = niters;
+        st.gs_passes = 2L * m * niters;
+        st.substeps = 1;
+        if (restarts) *restarts = nrest;
+        return st;
+    }
+};

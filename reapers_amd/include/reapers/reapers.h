@@ -69,6 +69,8 @@ using SumOps = DefImpl::SumOps<FpType>;
 
 #include "state.h"
 #include "schedule.h"
+#include "otoc.h"
+#include "batch.h"
 
 namespace Model {
 #include "fermions.h"

@@ -1,0 +1,123 @@
+"""Batched right-hand sides: problems of uneven size with 1, 2 or 4 states
+each in the block-state layout.  The batched apply against the oracle's
+per-state apply (every instantiated tiling), and the batched restarted
+Krylov evolution (HIP-graph replayed restarts) against the oracle's
+single-state evolution, the same evolution without the graph, and FP64."""
+import numpy as np
+import pytest
+
+from conftest import random_pauli_sum, random_state
+
+pytestmark = pytest.mark.gpu
+
+
+def problems(orc, rng, lens, fp):
+    out = []
+    for i, L in enumerate(lens):
+        if i % 2 == 0:
+            bits, coeff = orc.syk_terms(2 * L, 8.0, seed=500 + L + i, fp=fp)
+        else:
+            bits, coeff = random_pauli_sum(rng, L, 40)
+            # a Hermitian-free general sum exercises complex and multi subs
+            bits = np.concatenate([bits, bits[:5]])
+            coeff = np.concatenate([coeff, rng.standard_normal(5) * 1j])
+            if fp == 32:
+                coeff = coeff.astype(np.complex64).astype(np.complex128)
+        out.append((bits, coeff))
+    return out
+
+
+@pytest.mark.parametrize("fp", [32, 64])
+@pytest.mark.parametrize("nrhs", [1, 2, 4])
+@pytest.mark.parametrize("tb", [9, 10, 11])
+def test_batch_apply_vs_oracle(gpu, orc, fp, nrhs, tb):
+    if tb == 11 and (fp == 64 or nrhs > 1):
+        pytest.skip("tile bits 11 are instantiated for 8-byte elements only")
+    rng = np.random.default_rng(fp * 100 + nrhs * 10 + tb)
+    lens = [12, 14, 13, 12]
+    terms = problems(orc, rng, lens, fp)
+    b = gpu.Batch(lens, terms, nrhs=nrhs, kmax=4, fp=fp, tile_bits=tb)
+    assert b.info.tile_bits == tb
+    dt = np.complex64 if fp == 32 else np.complex128
+    states = [np.stack([random_state(rng, L, dt) for _ in range(nrhs)], 1) for L in lens]
+    b.load(states)
+    b.apply()
+    got = b.store()
+    worst = 0.0
+    for p, L in enumerate(lens):
+        bits, coeff = terms[p]
+        for s in range(nrhs):
+            ref = orc.apply(L, bits, coeff, states[p][:, s].astype(np.complex128), fp=64)
+            e = np.abs(got[p][:, s] - ref).max() / np.abs(ref).max()
+            worst = max(worst, e)
+    print("fp%d nrhs=%d tb=%d: max rel err %.2e" % (fp, nrhs, tb, worst), flush=True)
+    assert worst < (2e-6 if fp == 32 else 1e-13)
+
+
+def test_batch_autotune_picks_an_instantiated_tiling(gpu, orc):
+    rng = np.random.default_rng(5)
+    lens = [14, 16, 15]
+    b = gpu.Batch(lens, problems(orc, rng, lens, 32), nrhs=4, kmax=4, fp=32, autotune=True)
+    assert b.info.tile_bits in (9, 10)
+    assert b.info.grid >= 1
+
+
+@pytest.mark.parametrize("fp", [64, 32])
+def test_batch_evolve_vs_oracle(gpu, orc, fp):
+    rng = np.random.default_rng(11 + fp)
+    lens = [12, 14, 13, 12, 14]
+    terms = []
+    for i, L in enumerate(lens):
+        terms.append(orc.syk_terms(2 * L, 8.0, seed=900 + i, fp=fp))
+    nrhs, t, m = 4, 2.0, 10
+    dt = np.complex64 if fp == 32 else np.complex128
+    states = [np.stack([random_state(rng, L, dt) for _ in range(nrhs)], 1) for L in lens]
+    b = gpu.Batch(lens, terms, nrhs=nrhs, kmax=m, fp=fp)
+    b.load(states)
+    tol = 1e-13 if fp == 64 else 1e-7
+    st, rs = b.evolve(t, m=m, tol=tol)
+    got = b.store()
+    worst = 0.0
+    for p, L in enumerate(lens):
+        bits, coeff = terms[p]
+        for s in range(nrhs):
+            ref, _ = orc.krylov_evolve(L, bits, coeff, states[p][:, s].astype(np.complex128), t,
+                                       0.0, krydim=20, tol=1e-14)
+            worst = max(worst, np.linalg.norm(got[p][:, s] - ref))
+    print("fp%d batch evolve: max |err| %.2e, restarts %s, matvecs %d, %.1f ms" % (
+        fp, worst, rs.max(1).tolist(), st.matvecs, st.wall_ms), flush=True)
+    assert worst < (1e-11 if fp == 64 else 1e-5)
+
+
+def test_batch_graph_matches_eager(gpu, orc):
+    rng = np.random.default_rng(3)
+    lens = [13, 12, 14]
+    terms = [orc.syk_terms(2 * L, 8.0, seed=40 + L, fp=32) for L in lens]
+    states = [np.stack([random_state(rng, L, np.complex64) for _ in range(2)], 1) for L in lens]
+    out = []
+    for graph in (True, False, True):
+        b = gpu.Batch(lens, terms, nrhs=2, kmax=8, fp=32, tile_bits=10)
+        b.load(states)
+        b.evolve(3.0, m=8, tol=1e-7, graph=graph)
+        out.append(b.store())
+        b.close()
+    for p in range(len(lens)):
+        # graph replays and eager launches run the same kernels in the same order
+        assert np.array_equal(out[0][p], out[1][p])
+        assert np.array_equal(out[0][p], out[2][p])
+
+
+def test_batch_uneven_convergence(gpu, orc):
+    # a problem with H = 0 converges after one restart; the others keep going
+    rng = np.random.default_rng(8)
+    lens = [12, 12]
+    terms = [(np.array([0], np.uint64), np.array([0.5 + 0j])),
+             orc.syk_terms(24, 8.0, seed=77, fp=64)]
+    states = [np.stack([random_state(rng, 12) for _ in range(4)], 1) for _ in lens]
+    b = gpu.Batch(lens, terms, nrhs=4, kmax=6, fp=64)
+    b.load(states)
+    st, rs = b.evolve(4.0, m=6, tol=1e-12)
+    got = b.store()
+    # H = 0.5 Id: exp(-0.5 i t) psi, exact after one restart
+    assert np.abs(got[0] - np.exp(-2.0j) * states[0]).max() < 1e-13
+    assert rs[0].max() == 1 and rs[1].min() > 1
