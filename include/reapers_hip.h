@@ -119,6 +119,9 @@ int rp_terms_upload(rp_terms_t *out, int prec, int len, const uint64_t *bits,
  * apply in FP32 instead of FP64 (faster, one FP32 rounding per term instead
  * of one per output element). */
 #define RP_TERMS_ACC_NATIVE 1
+/* RP_TERMS_AUTOTUNE times the instantiated apply tilings (tile bits and
+ * workgroup size) on scratch vectors at upload and keeps the fastest */
+#define RP_TERMS_AUTOTUNE 2
 int rp_terms_upload_ex(rp_terms_t *out, int prec, int len, const uint64_t *bits,
                        const double *coeff, uint64_t nterms, int flags);
 /* DevSumOps::release (inc/gpuimpl.h:1087-1105) */
@@ -179,7 +182,8 @@ int rp_prof_query(int kernel_class, uint64_t *launches, double *total_ms);
 typedef struct rp_batch *rp_batch_t;
 typedef struct rp_graph *rp_graph_t;
 #define RP_BATCH_AUTOTUNE 1   /* time the instantiated tilings, keep the fastest */
-/* force the apply's tile bits b (9, 10 or 11; 11 only for 8-byte elements) */
+/* force the apply's tile bits b: the apply is instantiated for tiles of
+ * 32 KiB and 64 KiB, b = log2(32 KiB / (nrhs * element bytes)) or b + 1 */
 #define RP_BATCH_TILE_BITS(b) ((b) << 8)
 typedef struct {
     int prec, nprob, nrhs, kmax;
@@ -196,6 +200,7 @@ int rp_batch_create(rp_batch_t *out, int prec, int nprob, const int *len, const 
 int rp_batch_free(rp_batch_t b);
 int rp_batch_info(rp_batch_t b, rp_batch_info_t *info);
 int rp_batch_offsets(rp_batch_t b, uint64_t *off);              /* nprob + 1 entries */
+int rp_batch_groups(rp_batch_t b, uint64_t *groups);            /* G_b of each problem */
 int rp_batch_set_active(rp_batch_t b, const int *active_host);  /* nprob flags */
 /* the basis table V_0..V_{n-1} read by the Arnoldi step and lincomb */
 int rp_batch_set_basis(rp_batch_t b, void *const *V, int n);

@@ -22,6 +22,7 @@ RP_C64, RP_C128 = 0, 1
 RP_ROUND_NEAREST, RP_ROUND_TOWARD_ZERO = 0, 1
 RP_H2D, RP_D2H, RP_D2D = 0, 1, 2
 RP_KERNEL_APPLY, RP_KERNEL_GS, RP_KERNEL_OTHER = 0, 1, 2
+RP_TERMS_ACC_NATIVE, RP_TERMS_AUTOTUNE = 1, 2
 
 _c = ctypes
 _vp = _c.c_void_p
@@ -157,8 +158,9 @@ def host():
             "rph_batch_load": ([_vp, _vp], _c.c_int),
             "rph_batch_store": ([_vp, _vp], _c.c_int),
             "rph_batch_apply": ([_vp], _c.c_int),
-            "rph_batch_evolve": ([_vp, _dbl, _dbl, _c.c_int, _dbl, _c.c_int, _c.c_int,
+            "rph_batch_evolve": ([_vp, _dbl, _dbl, _c.c_int, _dbl, _c.c_int, _c.c_int, _vp,
                                   _c.POINTER(EvolveStats), _vp], _c.c_int),
+            "rph_set_upload_flags": ([_c.c_int], _c.c_int),
             "rph_state_inner": ([_vp, _vp, _c.POINTER(_dbl), _c.POINTER(_dbl)], _c.c_int),
             "rph_state_norm": ([_vp, _c.POINTER(_dbl)], _c.c_int),
             "rph_state_ground": ([_vp, _vp, _c.c_int, _dbl, _c.c_int, _c.POINTER(_dbl)], _c.c_int),
@@ -250,6 +252,12 @@ def ops_mul(a, b, fp=64):
 # ---------------------------------------------------------------------------
 # device objects (GPU required)
 # ---------------------------------------------------------------------------
+def set_upload_flags(flags):
+    """Flags of every later Hamiltonian upload (RP_TERMS_AUTOTUNE,
+    RP_TERMS_ACC_NATIVE)."""
+    hcheck(host().rph_set_upload_flags(flags))
+
+
 def init(device=0):
     check(hip().rp_init(device))
 
@@ -554,10 +562,13 @@ class Batch:
     def apply(self):
         hcheck(host().rph_batch_apply(self._h))
 
-    def evolve(self, t, m=12, beta=0.0, tol=0.0, max_iters=0, graph=True):
+    def evolve(self, t, m=12, beta=0.0, tol=0.0, max_iters=0, graph=True, schedule=None):
+        """Evolve every state; schedule: a Schedule (Krylov dimension per
+        restart, at most kmax) or None for the fixed m."""
         st = EvolveStats()
         rs = np.zeros(self.nprob * self.nrhs, np.int32)
         hcheck(host().rph_batch_evolve(self._h, t, beta, m, tol, max_iters, 1 if graph else 0,
+                                       schedule._h if schedule is not None else None,
                                        _c.byref(st), _ptr(rs)))
         return st, rs.reshape(self.nprob, self.nrhs)
 

@@ -535,20 +535,30 @@ int rph_batch_apply(void *h)
     });
 }
 
+// sched: a schedule handle (rph_sched_*_new) or null for the fixed m
 int rph_batch_evolve(void *h, double t, double beta, int m, double tol, int max_iters,
-                     int use_graph, rph_stats *out, int *restarts)
+                     int use_graph, void *sched, rph_stats *out, int *restarts)
 {
     return guarded([&] {
         Batch *b = (Batch *)h;
         std::vector<int> rs;
+        KrylovSchedule *sc = (Sched *)sched;
         auto t0 = std::chrono::steady_clock::now();
-        KrylovStats st = b->fp == 32 ? b->f->evolve(t, beta, m, tol, max_iters, use_graph != 0, &rs)
-                                     : b->d->evolve(t, beta, m, tol, max_iters, use_graph != 0, &rs);
+        KrylovStats st = b->fp == 32
+            ? b->f->evolve(t, beta, m, tol, max_iters, use_graph != 0, &rs, sc)
+            : b->d->evolve(t, beta, m, tol, max_iters, use_graph != 0, &rs, sc);
         double wall = std::chrono::duration<double, std::milli>(
                           std::chrono::steady_clock::now() - t0).count();
         fill_stats(out, st, wall);
         if (restarts) std::copy(rs.begin(), rs.end(), restarts);
     });
+}
+
+// flags of later term uploads (RP_TERMS_AUTOTUNE, RP_TERMS_ACC_NATIVE)
+int rph_set_upload_flags(int flags)
+{
+    GPUImpl::upload_flags() = flags;
+    return 0;
 }
 
 // schedule handles --------------------------------------------------------

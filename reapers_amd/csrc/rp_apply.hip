@@ -396,6 +396,79 @@ static int apply_impl(rp_terms *h, void *res, const void *vec, int accumulate)
 
 using namespace rp;
 
+static int upload_any(rp_terms *h, const uint64_t *bits, const double *coeff, uint64_t n)
+{
+    if (h->prec == RP_C128) return upload_impl<double, double>(h, bits, coeff, n);
+    if (h->acc64) return upload_impl<float, double>(h, bits, coeff, n);
+    return upload_impl<float, float>(h, bits, coeff, n);
+}
+
+static void free_tables(rp_terms *h)
+{
+    for (void **p : {&h->d_groups, &h->d_subs, &h->d_terms, &h->d_flat})
+        if (*p) { (void)hipFree(*p); *p = nullptr; }
+}
+
+static int apply_dispatch(rp_terms *h, void *res, const void *vec, int accumulate)
+{
+    if (h->prec == RP_C128) return apply_impl<double, double>(h, res, vec, accumulate);
+    if (h->acc64) return apply_impl<float, double>(h, res, vec, accumulate);
+    return apply_impl<float, float>(h, res, vec, accumulate);
+}
+
+// Time every instantiated tiling that fits len on scratch vectors (3
+// applies each after a warm-up) and keep the fastest.
+static int autotune(rp_terms *h, const uint64_t *bits, const double *coeff, uint64_t n)
+{
+    static const int cand[][2] = {{8, 128}, {10, 128}, {12, 512}, {13, 1024}};
+    Context &c = ctx();
+    const size_t bytes = (size_t)elem_bytes(h->prec) << h->len;
+    void *x = nullptr, *y = nullptr;
+    if (hipMalloc(&x, bytes) != hipSuccess || hipMalloc(&y, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        if (x) (void)hipFree(x);
+        return set_error(RP_ERR_OUT_OF_MEMORY, "rp_terms_upload_ex: no memory for autotuning");
+    }
+    int st = RP_OK;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    float best = 1e30f;
+    int btb = h->tile_bits, bnt = h->threads;
+    if (hipMemsetAsync(x, 0, bytes, c.stream) != hipSuccess ||
+        hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        st = set_error(RP_ERR_RUNTIME, "rp_terms_upload_ex: autotuning set-up failed");
+    for (auto &cd : cand) {
+        if (st) break;
+        if (cd[0] > h->len || (cd[0] == 13 && h->prec != RP_C64)) continue;
+        free_tables(h);
+        h->tile_bits = cd[0];
+        h->threads = cd[1];
+        if ((st = upload_any(h, bits, coeff, n))) break;
+        if ((st = apply_dispatch(h, y, x, 0))) break;
+        if (hipEventRecord(e0, c.stream) != hipSuccess) { st = RP_ERR_RUNTIME; break; }
+        for (int r = 0; r < 3 && !st; r++) st = apply_dispatch(h, y, x, 0);
+        float ms = 0;
+        if (st || hipEventRecord(e1, c.stream) != hipSuccess ||
+            hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+            if (!st) st = set_error(RP_ERR_RUNTIME, "rp_terms_upload_ex: autotuning timing failed");
+            break;
+        }
+        if (ms < best) { best = ms; btb = cd[0]; bnt = cd[1]; }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(c.stream);
+    (void)hipFree(x);
+    (void)hipFree(y);
+    if (st) return st;
+    if (btb != h->tile_bits || bnt != h->threads) {
+        free_tables(h);
+        h->tile_bits = btb;
+        h->threads = bnt;
+        st = upload_any(h, bits, coeff, n);
+    }
+    return st;
+}
+
 extern "C" {
 
 int rp_terms_upload_ex(rp_terms_t *out, int prec, int len, const uint64_t *bits,
@@ -408,15 +481,15 @@ int rp_terms_upload_ex(rp_terms_t *out, int prec, int len, const uint64_t *bits,
     RP_REQUIRE(valid_prec(prec), "bad precision");
     RP_REQUIRE(len >= 1 && len <= 32, "len must be in [1, 32]");
     RP_REQUIRE(nterms == 0 || (bits && coeff), "null term arrays");
-    RP_REQUIRE((flags & ~RP_TERMS_ACC_NATIVE) == 0, "unknown flags");
+    RP_REQUIRE((flags & ~(RP_TERMS_ACC_NATIVE | RP_TERMS_AUTOTUNE)) == 0, "unknown flags");
     rp_terms *h = new rp_terms;
     h->prec = prec;
     h->len = len;
     h->acc64 = (prec == RP_C128 || !(flags & RP_TERMS_ACC_NATIVE)) ? 1 : 0;
     choose_tiling(prec, len, &h->tile_bits, &h->threads);
-    if (prec == RP_C128) st = upload_impl<double, double>(h, bits, coeff, nterms);
-    else if (h->acc64) st = upload_impl<float, double>(h, bits, coeff, nterms);
-    else st = upload_impl<float, float>(h, bits, coeff, nterms);
+    st = upload_any(h, bits, coeff, nterms);
+    if (!st && (flags & RP_TERMS_AUTOTUNE) && h->tile_bits && h->nterms && len <= 31)
+        st = autotune(h, bits, coeff, nterms);
     if (st) {
         free_handle(h);
         return st;
@@ -461,9 +534,7 @@ int rp_apply_ops(rp_terms_t h, void *res, const void *vec, int accumulate)
     RP_REQUIRE(h, "null term handle");
     RP_REQUIRE(res && vec, "null vector");
     RP_REQUIRE(res != vec, "res and vec must be different buffers");
-    if (h->prec == RP_C128) return apply_impl<double, double>(h, res, vec, accumulate);
-    if (h->acc64) return apply_impl<float, double>(h, res, vec, accumulate);
-    return apply_impl<float, float>(h, res, vec, accumulate);
+    return apply_dispatch(h, res, vec, accumulate);
 }
 
 }  // extern "C"

@@ -530,9 +530,17 @@ void batch_release(rp_batch *b)
     delete b;
 }
 
-struct Tiling { int tb, nt; };
-// tilings the batched apply is instantiated for; the autotuner times them
-constexpr Tiling kTilings[] = {{9, 128}, {10, 256}, {11, 512}};
+// The apply is instantiated for two tile sizes per element type: 32 KiB
+// (256 threads) and 64 KiB (512 threads) of LDS per tile, i.e. tile bits
+// b = log2(32 KiB / element bytes) and b + 1, element = nrhs complex values.
+constexpr int ilog2c(int v) { return v <= 1 ? 0 : 1 + ilog2c(v / 2); }
+template <typename R, int NS> constexpr int small_tb() { return ilog2c(32768 / (2 * (int)sizeof(R) * NS)); }
+
+int tile_bits_for(int prec, int ns, int large)
+{
+    const int esz = (prec == RP_C64 ? 8 : 16) * ns;
+    return ilog2c(32768 / esz) + (large ? 1 : 0);
+}
 
 template <typename R, typename A, int TB, int NT, int NS>
 int launch_apply_t(rp_batch *b, void *res, const void *vec, int accumulate)
@@ -549,15 +557,9 @@ int launch_apply_t(rp_batch *b, void *res, const void *vec, int accumulate)
 template <typename R, typename A, int NS>
 int launch_apply_ns(rp_batch *b, void *res, const void *vec, int accumulate)
 {
-    switch (b->tile_bits) {
-    case 9: return launch_apply_t<R, A, 9, 128, NS>(b, res, vec, accumulate);
-    case 10: return launch_apply_t<R, A, 10, 256, NS>(b, res, vec, accumulate);
-    case 11:
-        if constexpr (sizeof(R) * NS <= 8)
-            return launch_apply_t<R, A, 11, 512, NS>(b, res, vec, accumulate);
-        break;
-    default: break;
-    }
+    constexpr int S = small_tb<R, NS>();
+    if (b->tile_bits == S) return launch_apply_t<R, A, S, 256, NS>(b, res, vec, accumulate);
+    if (b->tile_bits == S + 1) return launch_apply_t<R, A, S + 1, 512, NS>(b, res, vec, accumulate);
     return set_error(RP_ERR_RUNTIME, "batched apply: tiling b=%d not instantiated for %d states",
                      b->tile_bits, NS);
 }
@@ -578,12 +580,6 @@ int launch_apply(rp_batch *b, void *res, const void *vec, int accumulate)
         }
     }
     return set_error(RP_ERR_INVALID_ARGUMENT, "batched apply: nrhs must be 1, 2 or 4");
-}
-
-bool tiling_supported(int prec, int ns, int tb)
-{
-    const int esz = (prec == RP_C64 ? 8 : 16) * ns;
-    return !(tb == 11 && esz > 8);
 }
 
 // (re)build the group tables of every problem for tile bits tb
@@ -681,10 +677,9 @@ int rp_batch_create(rp_batch_t *out, int prec, int nprob, const int *len, const 
     RP_REQUIRE(kmax >= 2 && kmax <= 64, "kmax must be in [2, 64]");
     RP_REQUIRE((flags & ~(RP_BATCH_AUTOTUNE | 0xff00)) == 0, "unknown flags");
     const int forced_tb = (flags >> 8) & 0xff;
-    RP_REQUIRE(forced_tb == 0 || forced_tb == 9 || forced_tb == 10 || forced_tb == 11,
-               "forced tile bits must be 9, 10 or 11");
-    RP_REQUIRE(forced_tb == 0 || tiling_supported(prec, nrhs, forced_tb),
-               "tile bits 11 need 8-byte elements (complex<float>, nrhs = 1)");
+    RP_REQUIRE(forced_tb == 0 || forced_tb == tile_bits_for(prec, nrhs, 0) ||
+                   forced_tb == tile_bits_for(prec, nrhs, 1),
+               "forced tile bits must be log2(32 KiB or 64 KiB / (nrhs * element bytes))");
     int minlen = 32;
     for (int p = 0; p < nprob; p++) {
         RP_REQUIRE(len[p] >= 12 && len[p] <= 30, "every len must be in [12, 30]");
@@ -738,10 +733,16 @@ int rp_batch_create(rp_batch_t *out, int prec, int nprob, const int *len, const 
     RP_CHECK_HIP(hipStreamSynchronize(c.stream));
     b->V.assign(kmax + 2, nullptr);
     // default tiling: 2^10-amplitude tiles, 256 threads
-    int tb = 10, nt = 256;
-    if (forced_tb) {
-        for (const Tiling &T : kTilings)
-            if (T.tb == forced_tb) { tb = T.tb; nt = T.nt; }
+    // default tiling: 64 KiB tiles if every problem has that many amplitudes
+    int tb = tile_bits_for(prec, nrhs, 1), nt = 512;
+    if (tb > minlen || forced_tb == tile_bits_for(prec, nrhs, 0)) {
+        tb = tile_bits_for(prec, nrhs, 0);
+        nt = 256;
+    }
+    if (tb > minlen) {
+        batch_release(b);
+        return set_error(RP_ERR_INVALID_ARGUMENT, "rp_batch_create: problems of %d qubits are "
+                         "smaller than one %d-amplitude tile", minlen, 1 << tb);
     }
     if ((st = build_batch_tables(b, tb, nt, bits, coeff, term_off))) {
         batch_release(b);
@@ -761,9 +762,11 @@ int rp_batch_create(rp_batch_t *out, int prec, int nprob, const int *len, const 
         RP_CHECK_HIP(hipEventCreate(&e0));
         RP_CHECK_HIP(hipEventCreate(&e1));
         float best = 1e30f;
+        struct Tiling { int tb, nt; };
         Tiling pick{tb, nt};
-        for (const Tiling &T : kTilings) {
-            if (T.tb > minlen || !tiling_supported(prec, nrhs, T.tb)) continue;
+        for (int large = 0; large < 2; large++) {
+            const Tiling T{tile_bits_for(prec, nrhs, large), large ? 512 : 256};
+            if (T.tb > minlen) continue;
             if ((st = build_batch_tables(b, T.tb, T.nt, bits, coeff, term_off))) break;
             if ((st = launch_apply(b, y, x, 0))) break;   // warm-up
             RP_CHECK_HIP(hipEventRecord(e0, c.stream));
@@ -826,6 +829,13 @@ int rp_batch_offsets(rp_batch_t b, uint64_t *off)
 {
     RP_REQUIRE(b && off, "null argument");
     for (int p = 0; p <= b->nprob; p++) off[p] = b->off[p];
+    return RP_OK;
+}
+
+int rp_batch_groups(rp_batch_t b, uint64_t *groups)
+{
+    RP_REQUIRE(b && groups, "null argument");
+    for (int p = 0; p < b->nprob; p++) groups[p] = b->probs[p].g1 - b->probs[p].g0;
     return RP_OK;
 }
 

@@ -57,6 +57,15 @@ public:
             std::vector<u64> b1;
             std::vector<double> c1;
             h->export_terms(b1, c1);
+            double s1 = 0, s2 = 0;
+            for (size_t q = 0; q < b1.size(); q++) {
+                double a = std::abs(std::complex<double>(c1[2 * q], c1[2 * q + 1]));
+                s1 += a;
+                s2 += a * a;
+            }
+            pterms.push_back((long)b1.size());
+            ph_l1.push_back(s1);
+            ph_rms.push_back(std::sqrt(s2));
             bits.insert(bits.end(), b1.begin(), b1.end());
             coeff.insert(coeff.end(), c1.begin(), c1.end());
             toff.push_back((int64_t)bits.size());
@@ -66,6 +75,9 @@ public:
         RP_CALL(rp_batch_info(b, &info));
         off.resize(nprob + 1);
         RP_CALL(rp_batch_offsets(b, off.data()));
+        std::vector<uint64_t> groups(nprob);
+        RP_CALL(rp_batch_groups(b, groups.data()));
+        for (int p = 0; p < nprob; p++) ppasses.push_back((double)groups[p] + 1.0);
         const size_t nr = (size_t)nprob * ns;
         void *p;
         RP_CALL(rp_malloc(&p, nr * (kmax + 1) * kmax * 2 * sizeof(double)));
@@ -80,6 +92,7 @@ public:
     BatchedStates(const BatchedStates &) = delete;
     BatchedStates &operator=(const BatchedStates &) = delete;
     ~BatchedStates() {
+        for (auto &g : graphs) rp_graph_free(g.second);
         for (void *p : bufs) rp_free(p);
         for (void *p : {(void *)H_dev, (void *)nrm_dev, (void *)c_dev})
             if (p) rp_free(p);
@@ -102,21 +115,83 @@ public:
         std::swap(bufs[0], bufs[1]);
     }
 
-    // states = exp(-(beta + i t) H_p) states, restarted Krylov of dimension m
-    // per right-hand side; restarts[rhs] (optional) receives the restarts each
-    // right-hand side needed
+    // states = exp(-(beta + i t) H_p) states by restarted Krylov.  Without a
+    // schedule every restart has dimension m; with one, the sub-step count is
+    // the largest any state asks for, and each restart's dimension the
+    // largest the active states propose after the restart's first Arnoldi
+    // step (bounded by kmax); a restart whose update norm fails to shrink for
+    // some state rejects the schedule for the rest of the sub-step, which
+    // then continues with m.  restarts[rhs] (optional) receives the restarts
+    // each right-hand side needed.
     KrylovStats evolve(double t, double beta, int m, double tol, int max_iters = 0,
-                       bool use_graph = true, std::vector<int> *restarts = nullptr) {
-        using cd = std::complex<double>;
+                       bool use_graph = true, std::vector<int> *restarts = nullptr,
+                       KrylovSchedule *sched = nullptr) {
         if (m < 2 || m > kmax) ThrowException(InvalidArgument, "m", "must be in [2, kmax]");
-        KrylovStats st;
         const int nr = nprob * ns;
-        // buffers: V_0 .. V_m at bufs[1 .. m+1], the answer at bufs[m+2]
-        enlarge(m + 3);
-        std::vector<void *> V(bufs.begin() + 1, bufs.begin() + m + 2);
-        void *ans = bufs[m + 2];
+        KrylovStats st;
+        std::vector<int> rs(nr, 0);
+        int nsub = 1;
+        if (sched) {
+            for (int p = 0; p < nprob; p++) {
+                RestartInfo ri = base_info(p, t, beta, tol);
+                ri.tau = t;
+                ri.taub = beta;
+                nsub = std::max(nsub, std::min(64, sched->substeps(ri)));
+            }
+        }
+        for (int k = 0; k < nsub; k++)
+            evolve_once(t / nsub, beta / nsub, t, beta, m, tol, max_iters, use_graph, sched, k,
+                        nsub, st, rs);
+        st.substeps = nsub;
+        if (restarts) *restarts = rs;
+        return st;
+    }
+
+private:
+    std::map<std::pair<int, int>, rp_graph_t> graphs;   // Arnoldi steps [j0, j1)
+    std::vector<long> pterms;
+    std::vector<double> ph_rms, ph_l1, ppasses;
+
+    RestartInfo base_info(int p, double t, double beta, double tol) const {
+        RestartInfo ri;
+        ri.len = lens[p];
+        ri.nterms = pterms[p];
+        ri.h_rms = ph_rms[p];
+        ri.h_l1 = ph_l1[p];
+        ri.apply_passes = ppasses[p];
+        ri.t = t;
+        ri.beta = beta;
+        ri.tol = tol;
+        return ri;
+    }
+
+    void run_steps(int j0, int j1, bool use_graph) {
+        if (!use_graph) {
+            for (int j = j0; j < j1; j++) RP_CALL(rp_batch_arnoldi_step(b, j, H_dev));
+            return;
+        }
+        auto key = std::make_pair(j0, j1);
+        auto it = graphs.find(key);
+        if (it == graphs.end()) {
+            rp_graph_t g = nullptr;
+            RP_CALL(rp_batch_capture_arnoldi(b, j0, j1, H_dev, &g));
+            it = graphs.emplace(key, g).first;
+        }
+        RP_CALL(rp_graph_launch(it->second));
+    }
+
+    void evolve_once(double t, double beta, double t_all, double beta_all, int m_fixed,
+                     double tol, int max_iters, bool use_graph, KrylovSchedule *sched, int substep,
+                     int nsub, KrylovStats &st, std::vector<int> &rs) {
+        using cd = std::complex<double>;
+        const int nr = nprob * ns;
+        const int mmax = sched ? kmax : m_fixed;
+        // buffers: V_0 .. V_mmax at bufs[1 .. mmax+1], the answer at bufs[mmax+2]
+        enlarge(mmax + 3);
+        std::vector<void *> V(bufs.begin() + 1, bufs.begin() + mmax + 2);
+        void *ans = bufs[mmax + 2];
         RP_CALL(rp_memcpy(V[0], bufs[0], info.vec_bytes, RP_D2D));
-        RP_CALL(rp_batch_set_basis(b, V.data(), m + 1));
+        RP_CALL(rp_batch_set_basis(b, V.data(), mmax + 1));
         std::vector<int> act(nprob, 1);
         RP_CALL(rp_batch_set_active(b, act.data()));
         RP_CALL(rp_batch_normalize(b, 0, nrm_dev));
@@ -124,92 +199,117 @@ public:
         RP_CALL(rp_memcpy(beta0.data(), nrm_dev, nr * sizeof(double), RP_D2H));
         RP_CALL(rp_zero_vec(prec, ans, info.nbasis * ns));
         std::vector<char> done(nr, 0);
-        std::vector<int> nrest(nr, 0);
         for (int r = 0; r < nr; r++) done[r] = beta0[r] == 0;
-        rp_graph_t graph = nullptr;
-        if (use_graph) RP_CALL(rp_batch_capture_arnoldi(b, 0, m, H_dev, &graph));
         std::vector<dense::CMat> Hfull(nr);
-        std::vector<double> oldbeta(nr, 0.0);
+        std::vector<double> oldbeta(nr, 0.0), last(nr, 0.0), prev(nr, 0.0);
         const size_t hstride = (size_t)2 * (kmax + 1) * kmax;
         std::vector<double> Hh(hstride * nr), ch((size_t)2 * kmax * nr);
-        int niters = 0, off_blk = 0;
+        auto Hget = [&](int r, int i, int j) {
+            const double *q = Hh.data() + hstride * r + 2 * ((size_t)j * (kmax + 1) + i);
+            return cd(q[0], q[1]);
+        };
+        int niters = 0, off_blk = 0, mprev = 0;
+        bool use_sched = sched != nullptr;
         const cd z{-beta, -t};
-        try {
-            while (true) {
-                niters++;
-                for (int p = 0; p < nprob; p++) {
-                    act[p] = 0;
-                    for (int s = 0; s < ns; s++) act[p] |= !done[p * ns + s];
-                }
-                RP_CALL(rp_batch_set_active(b, act.data()));
-                if (graph) {
-                    RP_CALL(rp_graph_launch(graph));
-                } else {
-                    for (int j = 0; j < m; j++) RP_CALL(rp_batch_arnoldi_step(b, j, H_dev));
-                }
-                for (int p = 0; p < nprob; p++)
-                    if (act[p]) st.matvecs += (long)m * ns;
+        while (true) {
+            niters++;
+            for (int p = 0; p < nprob; p++) {
+                act[p] = 0;
+                for (int s = 0; s < ns; s++) act[p] |= !done[p * ns + s];
+            }
+            RP_CALL(rp_batch_set_active(b, act.data()));
+            int m = m_fixed;
+            if (use_sched) {
+                // first Arnoldi step, then the proposals from alpha0, beta0
+                run_steps(0, 1, use_graph);
                 RP_CALL(rp_memcpy(Hh.data(), H_dev, Hh.size() * sizeof(double), RP_D2H));
-                auto t0 = std::chrono::steady_clock::now();
-                std::fill(ch.begin(), ch.end(), 0.0);
-                std::vector<char> now_done(done);
-#pragma omp parallel for schedule(dynamic)
+                m = 2;
                 for (int r = 0; r < nr; r++) {
                     if (done[r]) continue;
-                    const double *Hr = Hh.data() + hstride * r;
-                    auto Hm = [&](int i, int j) {
-                        const double *q = Hr + 2 * ((size_t)j * (kmax + 1) + i);
-                        return cd(q[0], q[1]);
-                    };
-                    dense::CMat &F = Hfull[r];
-                    F.grow(off_blk + m);
-                    if (off_blk) F(off_blk, off_blk - 1) = oldbeta[r];
-                    for (int i = 0; i < m; i++)
-                        for (int j = 0; j < m; j++) F(off_blk + i, off_blk + j) = i <= j + 1 ? Hm(i, j) : 0.0;
-                    dense::CMat tH = F;
-                    for (auto &v : tH.a) v *= z;
-                    dense::CMat E = dense::expm(tH);
-                    double nrm2 = 0;
-                    for (int i = 0; i < m; i++) {
-                        cd e = E(off_blk + i, 0);
-                        cd c = beta0[r] * e;
-                        ch[(size_t)2 * kmax * r + 2 * i] = c.real();
-                        ch[(size_t)2 * kmax * r + 2 * i + 1] = c.imag();
-                        nrm2 += std::norm(e);
-                    }
-                    oldbeta[r] = Hm(m, m - 1).real();
-                    nrest[r] = niters;
-                    // a zero norm (inverse 0 on the device) is the Arnoldi
-                    // breakdown: the Krylov space is invariant and the
-                    // answer exact (inc/algo.h:356)
-                    bool brk = false;
-                    for (int j = 0; j < m; j++) brk = brk || Hm(j + 1, j) == 0.0;
-                    if (brk || std::sqrt(nrm2) < tol || (max_iters && niters >= max_iters))
-                        now_done[r] = 1;
+                    RestartInfo ri = base_info(r / ns, t_all, beta_all, tol);
+                    ri.tau = t;
+                    ri.taub = beta;
+                    ri.substep = substep;
+                    ri.nsub = nsub;
+                    ri.restart = niters - 1;
+                    ri.last_update = last[r];
+                    ri.prev_update = prev[r];
+                    ri.prev_krydim = mprev;
+                    ri.alpha0 = Hget(r, 0, 0).real();
+                    ri.beta0 = Hget(r, 1, 0).real();
+                    m = std::max(m, sched->krydim(ri));
                 }
-                st.host_ms += std::chrono::duration<double, std::milli>(
-                                  std::chrono::steady_clock::now() - t0).count();
-                RP_CALL(rp_memcpy(c_dev, ch.data(), ch.size() * sizeof(double), RP_H2D));
-                RP_CALL(rp_batch_lincomb(b, ans, m, c_dev, 2 * kmax));
-                done = now_done;
-                off_blk += m;
-                bool all = true;
-                for (char d : done) all = all && d;
-                if (all) break;
-                // restart from V_m: the graph's pointers stay fixed, so copy
-                RP_CALL(rp_memcpy(V[0], V[m], info.vec_bytes, RP_D2D));
+                m = std::min(m, kmax);
+                run_steps(1, m, use_graph);
+            } else {
+                run_steps(0, m, use_graph);
             }
-        } catch (...) {
-            if (graph) rp_graph_free(graph);
-            throw;
+            for (int p = 0; p < nprob; p++)
+                if (act[p]) st.matvecs += (long)m * ns;
+            st.gs_passes += 2L * m;
+            RP_CALL(rp_memcpy(Hh.data(), H_dev, Hh.size() * sizeof(double), RP_D2H));
+            auto t0 = std::chrono::steady_clock::now();
+            std::fill(ch.begin(), ch.end(), 0.0);
+            std::vector<char> now_done(done);
+            std::vector<double> nrm(nr, 0.0);
+#pragma omp parallel for schedule(dynamic)
+            for (int r = 0; r < nr; r++) {
+                if (done[r]) continue;
+                dense::CMat &F = Hfull[r];
+                F.grow(off_blk + m);
+                if (off_blk) F(off_blk, off_blk - 1) = oldbeta[r];
+                for (int i = 0; i < m; i++)
+                    for (int j = 0; j < m; j++)
+                        F(off_blk + i, off_blk + j) = i <= j + 1 ? Hget(r, i, j) : 0.0;
+                dense::CMat tH = F;
+                for (auto &v : tH.a) v *= z;
+                dense::CMat E = dense::expm(tH);
+                double nrm2 = 0;
+                for (int i = 0; i < m; i++) {
+                    cd e = E(off_blk + i, 0);
+                    cd c = beta0[r] * e;
+                    ch[(size_t)2 * kmax * r + 2 * i] = c.real();
+                    ch[(size_t)2 * kmax * r + 2 * i + 1] = c.imag();
+                    nrm2 += std::norm(e);
+                }
+                nrm[r] = std::sqrt(nrm2);
+                oldbeta[r] = Hget(r, m, m - 1).real();
+                // a zero norm (inverse 0 on the device) is the Arnoldi
+                // breakdown: the Krylov space is invariant and the answer
+                // exact (inc/algo.h:356)
+                bool brk = false;
+                for (int j = 0; j < m; j++) brk = brk || Hget(r, j + 1, j) == 0.0;
+                if (brk || nrm[r] < tol || (max_iters && niters >= max_iters)) now_done[r] = 1;
+            }
+            st.host_ms += std::chrono::duration<double, std::milli>(
+                              std::chrono::steady_clock::now() - t0).count();
+            RP_CALL(rp_memcpy(c_dev, ch.data(), ch.size() * sizeof(double), RP_H2D));
+            RP_CALL(rp_batch_lincomb(b, ans, m, c_dev, 2 * kmax));
+            bool all = true, grew = false;
+            for (int r = 0; r < nr; r++) {
+                if (done[r]) continue;
+                rs[r]++;
+                if (niters > 1 && nrm[r] >= last[r]) grew = true;
+                prev[r] = last[r];
+                last[r] = nrm[r];
+            }
+            done = now_done;
+            for (char d : done) all = all && d;
+            if (all) break;
+            if (use_sched && grew) {
+                use_sched = false;
+                st.rejected++;
+            }
+            off_blk += m;
+            mprev = m;
+            st.restarts++;
+            // restart from V_m: the graphs' pointers stay fixed, so copy
+            RP_CALL(rp_memcpy(V[0], V[m], info.vec_bytes, RP_D2D));
         }
-        if (graph) RP_CALL(rp_graph_free(graph));
-        std::swap(bufs[0], bufs[m + 2]);
+        st.restarts++;
+        std::swap(bufs[0], bufs[mmax + 2]);
         RP_CALL(rp_synchronize());
-        st.restarts = niters;
-        st.gs_passes = 2L * m * niters;
-        st.substeps = 1;
-        if (restarts) *restarts = nrest;
-        return st;
     }
+
+public:
 };

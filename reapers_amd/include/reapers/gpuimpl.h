@@ -149,6 +149,15 @@ public:
 
     static void synchronize() { RP_CALL(rp_synchronize()); }
 
+    // flags of every later term upload: RP_TERMS_AUTOTUNE times the apply
+    // tilings once per Hamiltonian, RP_TERMS_ACC_NATIVE accumulates FP32
+    // applies in FP32 (no counterpart in the reference, whose kernel has one
+    // launch shape, src/krnl.cu:236-246)
+    static int &upload_flags() {
+        static int flags = 0;
+        return flags;
+    }
+
     template<RealScalar FpType>
     static void zero_vec(size_t size, BufType<FpType> v) {
         RP_CALL(rp_zero_vec(internal::rp_prec<FpType>(), v.ptr, size));
@@ -290,8 +299,8 @@ inline rp_terms_t DevSumOps<FpType>::device_terms(int len) const {
     std::vector<u64> bits;
     std::vector<double> coeff;
     this->export_terms(bits, coeff);
-    RP_CALL(rp_terms_upload(&handle, internal::rp_prec<FpType>(), len, bits.data(), coeff.data(),
-                            bits.size()));
+    RP_CALL(rp_terms_upload_ex(&handle, internal::rp_prec<FpType>(), len, bits.data(),
+                               coeff.data(), bits.size(), GPUImpl::upload_flags()));
     handle_len = len;
     return handle;
 }

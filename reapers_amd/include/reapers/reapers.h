@@ -18,6 +18,7 @@
 #include <cstring>
 #include <exception>
 #include <iostream>
+#include <map>
 #include <memory>
 #include <random>
 #include <set>

@@ -84,3 +84,26 @@ def test_merged_flips_dense(gpu, orc):
     coeff = rng.standard_normal(64) + 1j * rng.standard_normal(64)
     psi = random_state(rng, length, np.complex128)
     _check(orc, length, bits, coeff, psi)
+
+
+@pytest.mark.parametrize("fp,L", [(32, 16), (64, 14), (32, 21)])
+def test_autotuned_upload_matches_oracle(gpu, orc, fp, L):
+    """RP_TERMS_AUTOTUNE times the instantiated tilings at upload; whichever
+    it keeps must give the oracle's result."""
+    rng = np.random.default_rng(L + fp)
+    bits, coeff = orc.syk_terms(2 * L, 8.0, seed=70 + L, fp=fp)
+    psi = random_state(rng, L)
+    gpu.set_upload_flags(gpu.RP_TERMS_AUTOTUNE)
+    try:
+        ham = gpu.Hamiltonian(L, bits, coeff, fp=fp)
+        s = gpu.DeviceState(L, fp=fp)
+        s.load(psi.astype(np.complex64 if fp == 32 else np.complex128))
+        s.apply(ham)
+        info = ham.info()
+    finally:
+        gpu.set_upload_flags(0)
+    ref = orc.apply(L, bits, coeff, psi, fp=64)
+    got = s.store().astype(np.complex128)
+    print("autotuned L=%d fp%d: tile bits %d" % (L, fp, info.tile_bits), flush=True)
+    assert info.tile_bits in (8, 10, 12, 13)
+    assert np.abs(got - ref).max() < (1e-5 if fp == 32 else 1e-12) * np.abs(ref).max()

@@ -27,14 +27,18 @@ def problems(orc, rng, lens, fp):
     return out
 
 
+def tile_bits(fp, nrhs, large):
+    esz = (8 if fp == 32 else 16) * nrhs
+    return int(np.log2(32768 // esz)) + large
+
+
 @pytest.mark.parametrize("fp", [32, 64])
 @pytest.mark.parametrize("nrhs", [1, 2, 4])
-@pytest.mark.parametrize("tb", [9, 10, 11])
-def test_batch_apply_vs_oracle(gpu, orc, fp, nrhs, tb):
-    if tb == 11 and (fp == 64 or nrhs > 1):
-        pytest.skip("tile bits 11 are instantiated for 8-byte elements only")
+@pytest.mark.parametrize("large", [0, 1])
+def test_batch_apply_vs_oracle(gpu, orc, fp, nrhs, large):
+    tb = tile_bits(fp, nrhs, large)
     rng = np.random.default_rng(fp * 100 + nrhs * 10 + tb)
-    lens = [12, 14, 13, 12]
+    lens = [14, 16, 15, 14] if tb > 12 else [12, 14, 13, 12]
     terms = problems(orc, rng, lens, fp)
     b = gpu.Batch(lens, terms, nrhs=nrhs, kmax=4, fp=fp, tile_bits=tb)
     assert b.info.tile_bits == tb
@@ -58,7 +62,7 @@ def test_batch_autotune_picks_an_instantiated_tiling(gpu, orc):
     rng = np.random.default_rng(5)
     lens = [14, 16, 15]
     b = gpu.Batch(lens, problems(orc, rng, lens, 32), nrhs=4, kmax=4, fp=32, autotune=True)
-    assert b.info.tile_bits in (9, 10)
+    assert b.info.tile_bits in (tile_bits(32, 4, 0), tile_bits(32, 4, 1))
     assert b.info.grid >= 1
 
 
@@ -96,7 +100,7 @@ def test_batch_graph_matches_eager(gpu, orc):
     states = [np.stack([random_state(rng, L, np.complex64) for _ in range(2)], 1) for L in lens]
     out = []
     for graph in (True, False, True):
-        b = gpu.Batch(lens, terms, nrhs=2, kmax=8, fp=32, tile_bits=10)
+        b = gpu.Batch(lens, terms, nrhs=2, kmax=8, fp=32, tile_bits=tile_bits(32, 2, 0))
         b.load(states)
         b.evolve(3.0, m=8, tol=1e-7, graph=graph)
         out.append(b.store())
@@ -121,3 +125,28 @@ def test_batch_uneven_convergence(gpu, orc):
     # H = 0.5 Id: exp(-0.5 i t) psi, exact after one restart
     assert np.abs(got[0] - np.exp(-2.0j) * states[0]).max() < 1e-13
     assert rs[0].max() == 1 and rs[1].min() > 1
+
+
+def test_batch_with_proposer_schedule(gpu, orc):
+    """The batch with the trained proposer choosing each restart's dimension
+    (the largest the active states propose) reaches the oracle's result."""
+    from reapers_amd import proposer
+    rng = np.random.default_rng(21)
+    lens = [12, 13, 14]
+    terms = [orc.syk_terms(2 * L, 8.0, seed=60 + L, fp=64) for L in lens]
+    states = [np.stack([random_state(rng, L) for _ in range(4)], 1) for L in lens]
+    sched = gpu.Schedule.from_mlp(proposer.load_weights(), max_dim=16)
+    b = gpu.Batch(lens, terms, nrhs=4, kmax=16, fp=64)
+    b.load(states)
+    t, tol = 4.0, 1e-10
+    st, rs = b.evolve(t, m=12, tol=tol, schedule=sched)
+    got = b.store()
+    worst = 0.0
+    for p, L in enumerate(lens):
+        for s in range(4):
+            ref, _ = orc.krylov_evolve(L, *terms[p], states[p][:, s], t, 0.0, krydim=20,
+                                       tol=1e-14)
+            worst = max(worst, np.linalg.norm(got[p][:, s] - ref))
+    print("batch + proposer: err %.2e, matvecs %d, restarts %s, rejected %d" % (
+        worst, st.matvecs, rs.max(1).tolist(), st.rejected), flush=True)
+    assert worst < 10 * tol
