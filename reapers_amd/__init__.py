@@ -40,7 +40,7 @@ class RpError(RuntimeError):
 class TermsInfo(_c.Structure):
     _fields_ = [("prec", _c.c_int), ("len", _c.c_int), ("tile_bits", _c.c_int),
                 ("nterms", _u64), ("nflips", _u64), ("ngroups", _u64),
-                ("bytes_per_apply", _u64)]
+                ("bytes_per_apply", _u64), ("threads", _c.c_int), ("depth", _c.c_int)]
 
 
 class EvolveStats(_c.Structure):
@@ -88,6 +88,8 @@ def hip():
             "rp_terms_upload": [_c.POINTER(_vp), _c.c_int, _c.c_int, _vp, _vp, _u64],
             "rp_terms_free": [_vp],
             "rp_terms_info": [_vp, _c.POINTER(TermsInfo)],
+            "rp_terms_set_tiling": [_vp, _c.c_int, _c.c_int, _c.c_int],
+            "rp_batch_groups": [_vp, _vp],
             "rp_apply_ops": [_vp, _vp, _vp, _c.c_int],
             "rp_gs_pass": [_c.c_int, _vp, _c.c_int, _vp, _u64, _vp, _c.POINTER(_dbl)],
             "rp_lincomb": [_c.c_int, _vp, _vp, _c.c_int, _vp, _u64],

@@ -50,9 +50,12 @@ struct rp_terms {
     int len = 0;
     int tile_bits = 0;       // 0: small kernel (one thread per amplitude)
     int threads = 0;         // workgroup size of the tiled kernel
+    int depth = 2;           // LDS tiles in the tiled kernel's pipeline
     int acc64 = 1;           // accumulate in FP64
     uint64_t nterms = 0, nflips = 0, ngroups = 0, nsubs = 0;
     void *d_groups = nullptr, *d_subs = nullptr, *d_terms = nullptr, *d_flat = nullptr;
+    std::vector<uint64_t> raw_bits;   // the uploaded term list, for re-tiling
+    std::vector<double> raw_coeff;
 };
 
 namespace rp {
@@ -87,10 +90,10 @@ k_apply_small(const typename CxA<R>::T *__restrict__ in, typename CxA<R>::T *__r
     out[m] = o;
 }
 
-template <typename R, int TB, int NT>
+template <typename R, int TB, int NT, int D = 2>
 constexpr int apply_waves_per_eu()
 {
-    constexpr int lds = 2 * (1 << TB) * 2 * (int)sizeof(R);
+    constexpr int lds = D * (1 << TB) * 2 * (int)sizeof(R);
     constexpr int wg_by_lds = (160 * 1024) / lds;
     constexpr int wg_by_waves = 1024 / NT;
     constexpr int wg = wg_by_lds < wg_by_waves ? wg_by_lds : wg_by_waves;
@@ -234,8 +237,62 @@ __device__ __forceinline__ void apply_group(const GroupRuns &G, const typename C
 #undef RP_SUB_PROLOGUE
 }
 
-template <typename R, typename A, int TB, int NT>
-__global__ void __launch_bounds__(NT, (apply_waves_per_eu<R, TB, NT>()))
+// s_waitcnt immediate that waits until at most N vector memory operations
+// of this wave are outstanding and all of its LDS operations have completed
+// (gfx9 encoding: vmcnt in bits 3:0 and 15:14, expcnt 6:4 left at its
+// maximum, lgkmcnt 11:8 = 0)
+template <int N>
+__device__ inline void wait_vmcnt_lds()
+{
+    static_assert(N >= 0 && N < 64, "vmcnt range");
+    __builtin_amdgcn_s_waitcnt((N & 0xF) | ((N >> 4) << 14) | (0x7 << 4));
+}
+
+// Workgroup barrier without the release/acquire fences of __syncthreads(),
+// whose release would drain every outstanding global->LDS load.  The caller
+// has waited for its own loads into the tile about to be read (vmcnt) and
+// for its own LDS reads of the tile about to be overwritten (lgkmcnt 0); the
+// compiler barriers keep memory operations on their side.
+__device__ inline void tile_barrier()
+{
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
+// Pipeline of D LDS tiles: the input tiles of the next D-1 groups are in
+// flight while a group is applied.  Phase K of an unrolled round uses the
+// statically named buffer K, so the compiler sees which DMA writes which
+// array and does not drain every load before each LDS read.
+template <typename R, typename A, int TB, int NT, int D, int K>
+__device__ __forceinline__ void pipe_phase(
+    int g, int ngroups, const typename CxA<R>::T *__restrict__ in,
+    const GroupRuns *__restrict__ groups, const SubRec<A> *__restrict__ subs,
+    const TermRec<A> *__restrict__ terms, uint32_t t0, uint32_t tid,
+    typename CxA<R>::T *b0, typename CxA<R>::T *b1, typename CxA<R>::T *b2,
+    typename CxA<R>::T *b3, A (&accr)[(1 << TB) / NT], A (&acci)[(1 << TB) / NT])
+{
+    using T = typename CxA<R>::T;
+    constexpr int PT = ((1 << TB) * (int)sizeof(T)) / (16 * NT);   // DMA ops per thread per tile
+    T *const bufs[4] = {b0, b1, b2, b3};
+    // this thread's share of group g's tile has landed once at most the
+    // DMAs issued after it (groups g+1 .. g+D-2) are outstanding; after the
+    // barrier every share is visible and group g-1's buffer is no longer read
+    if constexpr (D == 2) {
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+    } else {
+        if (g + D - 2 < ngroups) wait_vmcnt_lds<(D - 2) * PT>();
+        else __builtin_amdgcn_s_waitcnt(0);
+        tile_barrier();
+    }
+    if (g + D - 1 < ngroups)
+        tile_dma<T, TB, NT>(in, t0 ^ (groups[g + D - 1].xh << TB), bufs[(K + D - 1) % D]);
+    apply_group<R, A, TB, NT>(groups[g], bufs[K], subs, terms, t0, tid, accr, acci);
+}
+
+template <typename R, typename A, int TB, int NT, int D>
+__global__ void __launch_bounds__(NT, (apply_waves_per_eu<R, TB, NT, D>()))
 k_apply_tiled(const typename CxA<R>::T *__restrict__ in, typename CxA<R>::T *__restrict__ out,
               const GroupRuns *__restrict__ groups, int ngroups, const SubRec<A> *__restrict__ subs,
               const TermRec<A> *__restrict__ terms, int accumulate)
@@ -245,10 +302,12 @@ k_apply_tiled(const typename CxA<R>::T *__restrict__ in, typename CxA<R>::T *__r
     constexpr int RP = TILE / NT;          // amplitudes per thread
     constexpr int LB = NT == 1024 ? 10 : NT == 512 ? 9 : NT == 256 ? 8 : 7;
     static_assert((1 << LB) == NT, "NT must be a power of two in [128, 1024]");
-    // two separate LDS tiles: while one is applied, the next group's input
-    // tile streams into the other
+    static_assert(D >= 2 && D <= 4, "2 to 4 LDS tiles");
+    // D separate LDS tiles (the unused ones have one element)
     __shared__ T tile0[TILE];
     __shared__ T tile1[TILE];
+    __shared__ T tile2[D > 2 ? TILE : 1];
+    __shared__ T tile3[D > 3 ? TILE : 1];
 
     // XCD-aware numbering: consecutive workgroup ids go to different XCDs,
     // so give each XCD a contiguous range of tiles
@@ -261,19 +320,26 @@ k_apply_tiled(const typename CxA<R>::T *__restrict__ in, typename CxA<R>::T *__r
 #pragma unroll
     for (int r = 0; r < RP; r++) { accr[r] = 0; acci[r] = 0; }
 
-    if (ngroups > 0) tile_dma<T, TB, NT>(in, t0 ^ (groups[0].xh << TB), tile0);
-    for (int g = 0; g < ngroups; g += 2) {
-        // this thread's share of tile g has landed; the barrier makes every
-        // share visible and guarantees nobody still reads the other tile
-        __builtin_amdgcn_s_waitcnt(0);
-        __syncthreads();
-        if (g + 1 < ngroups) tile_dma<T, TB, NT>(in, t0 ^ (groups[g + 1].xh << TB), tile1);
-        apply_group<R, A, TB, NT>(groups[g], tile0, subs, terms, t0, tid, accr, acci);
+    T *const ring[4] = {tile0, tile1, tile2, tile3};
+#pragma unroll
+    for (int q = 0; q < D - 1; q++)
+        if (q < ngroups) tile_dma<T, TB, NT>(in, t0 ^ (groups[q].xh << TB), ring[q]);
+    for (int g = 0; g < ngroups; g += D) {
+        pipe_phase<R, A, TB, NT, D, 0>(g, ngroups, in, groups, subs, terms, t0, tid, tile0, tile1,
+                                       tile2, tile3, accr, acci);
         if (g + 1 >= ngroups) break;
-        __builtin_amdgcn_s_waitcnt(0);
-        __syncthreads();
-        if (g + 2 < ngroups) tile_dma<T, TB, NT>(in, t0 ^ (groups[g + 2].xh << TB), tile0);
-        apply_group<R, A, TB, NT>(groups[g + 1], tile1, subs, terms, t0, tid, accr, acci);
+        pipe_phase<R, A, TB, NT, D, 1>(g + 1, ngroups, in, groups, subs, terms, t0, tid, tile0,
+                                       tile1, tile2, tile3, accr, acci);
+        if constexpr (D > 2) {
+            if (g + 2 >= ngroups) break;
+            pipe_phase<R, A, TB, NT, D, 2>(g + 2, ngroups, in, groups, subs, terms, t0, tid, tile0,
+                                           tile1, tile2, tile3, accr, acci);
+        }
+        if constexpr (D > 3) {
+            if (g + 3 >= ngroups) break;
+            pipe_phase<R, A, TB, NT, D, 3>(g + 3, ngroups, in, groups, subs, terms, t0, tid, tile0,
+                                           tile1, tile2, tile3, accr, acci);
+        }
     }
 #pragma unroll
     for (int r = 0; r < RP; r++) {
@@ -289,6 +355,24 @@ k_apply_tiled(const typename CxA<R>::T *__restrict__ in, typename CxA<R>::T *__r
         }
         out[m] = o;
     }
+}
+
+constexpr int tiling_key(int tb, int nt, int d) { return tb * 100000 + nt * 10 + d; }
+
+// the tilings the tiled kernel is instantiated for: {tile bits, threads,
+// pipeline depth, element bytes (0: both precisions)}
+struct TilingDesc { int tb, nt, depth, esz; };
+constexpr TilingDesc kTilings[] = {
+    {8, 128, 2, 0}, {10, 128, 2, 0}, {11, 256, 4, 16}, {12, 512, 2, 0},
+    {12, 512, 3, 8}, {12, 512, 4, 8}, {13, 1024, 2, 8}};
+
+static bool tiling_instantiated(int prec, int tb, int nt, int d)
+{
+    for (const TilingDesc &t : kTilings)
+        if (t.tb == tb && t.nt == nt && t.depth == d &&
+            (t.esz == 0 || t.esz == (int)elem_bytes(prec)))
+            return true;
+    return false;
 }
 
 // tile bits b and workgroup size: 8 amplitudes per thread, two 64 KiB LDS
@@ -352,12 +436,12 @@ static void free_handle(rp_terms *h)
     delete h;
 }
 
-template <typename R, typename A, int TB, int NT>
+template <typename R, typename A, int TB, int NT, int D = 2>
 static int launch_tiled(rp_terms *h, void *res, const void *vec, int accumulate)
 {
     using T = typename CxA<R>::T;
     const uint64_t grid = (1ull << h->len) >> TB;
-    RP_LAUNCH(RP_KERNEL_APPLY, (k_apply_tiled<R, A, TB, NT>), dim3((unsigned)grid), dim3(NT), 0,
+    RP_LAUNCH(RP_KERNEL_APPLY, (k_apply_tiled<R, A, TB, NT, D>), dim3((unsigned)grid), dim3(NT), 0,
               (const T *)vec, (T *)res, (const GroupRuns *)h->d_groups, (int)h->ngroups,
               (const SubRec<A> *)h->d_subs, (const TermRec<A> *)h->d_terms, accumulate);
     return RP_OK;
@@ -379,17 +463,26 @@ static int apply_impl(rp_terms *h, void *res, const void *vec, int accumulate)
                   (int)h->nterms, accumulate);
         return RP_OK;
     }
-    switch (h->tile_bits * 10000 + h->threads) {
-    case 80128: return launch_tiled<R, A, 8, 128>(h, res, vec, accumulate);
-    case 100128: return launch_tiled<R, A, 10, 128>(h, res, vec, accumulate);
-    case 120512: return launch_tiled<R, A, 12, 512>(h, res, vec, accumulate);
-    case 131024:
+    switch (tiling_key(h->tile_bits, h->threads, h->depth)) {
+    case tiling_key(8, 128, 2): return launch_tiled<R, A, 8, 128>(h, res, vec, accumulate);
+    case tiling_key(10, 128, 2): return launch_tiled<R, A, 10, 128>(h, res, vec, accumulate);
+    case tiling_key(12, 512, 2): return launch_tiled<R, A, 12, 512>(h, res, vec, accumulate);
+    case tiling_key(11, 256, 4):
+        if constexpr (sizeof(R) == 8) return launch_tiled<R, A, 11, 256, 4>(h, res, vec, accumulate);
+        break;
+    case tiling_key(12, 512, 3):
+        if constexpr (sizeof(R) == 4) return launch_tiled<R, A, 12, 512, 3>(h, res, vec, accumulate);
+        break;
+    case tiling_key(12, 512, 4):
+        if constexpr (sizeof(R) == 4) return launch_tiled<R, A, 12, 512, 4>(h, res, vec, accumulate);
+        break;
+    case tiling_key(13, 1024, 2):
         if constexpr (sizeof(R) == 4) return launch_tiled<R, A, 13, 1024>(h, res, vec, accumulate);
         break;
     default: break;
     }
-    return set_error(RP_ERR_RUNTIME, "rp_apply_ops: unsupported tiling b=%d threads=%d",
-                     h->tile_bits, h->threads);
+    return set_error(RP_ERR_RUNTIME, "rp_apply_ops: unsupported tiling b=%d threads=%d depth=%d",
+                     h->tile_bits, h->threads, h->depth);
 }
 
 }  // namespace rp
@@ -416,11 +509,20 @@ static int apply_dispatch(rp_terms *h, void *res, const void *vec, int accumulat
     return apply_impl<float, float>(h, res, vec, accumulate);
 }
 
+// rebuild the device tables for another tiling (from the stored term list)
+static int retile(rp_terms *h, int tb, int nt, int d)
+{
+    free_tables(h);
+    h->tile_bits = tb;
+    h->threads = nt;
+    h->depth = d;
+    return upload_any(h, h->raw_bits.data(), h->raw_coeff.data(), h->raw_bits.size());
+}
+
 // Time every instantiated tiling that fits len on scratch vectors (3
 // applies each after a warm-up) and keep the fastest.
-static int autotune(rp_terms *h, const uint64_t *bits, const double *coeff, uint64_t n)
+static int autotune(rp_terms *h)
 {
-    static const int cand[][2] = {{8, 128}, {10, 128}, {12, 512}, {13, 1024}};
     Context &c = ctx();
     const size_t bytes = (size_t)elem_bytes(h->prec) << h->len;
     void *x = nullptr, *y = nullptr;
@@ -432,17 +534,14 @@ static int autotune(rp_terms *h, const uint64_t *bits, const double *coeff, uint
     int st = RP_OK;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     float best = 1e30f;
-    int btb = h->tile_bits, bnt = h->threads;
+    TilingDesc pick{h->tile_bits, h->threads, h->depth, 0};
     if (hipMemsetAsync(x, 0, bytes, c.stream) != hipSuccess ||
         hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
         st = set_error(RP_ERR_RUNTIME, "rp_terms_upload_ex: autotuning set-up failed");
-    for (auto &cd : cand) {
+    for (const TilingDesc &cd : kTilings) {
         if (st) break;
-        if (cd[0] > h->len || (cd[0] == 13 && h->prec != RP_C64)) continue;
-        free_tables(h);
-        h->tile_bits = cd[0];
-        h->threads = cd[1];
-        if ((st = upload_any(h, bits, coeff, n))) break;
+        if (cd.tb > h->len || !tiling_instantiated(h->prec, cd.tb, cd.nt, cd.depth)) continue;
+        if ((st = retile(h, cd.tb, cd.nt, cd.depth))) break;
         if ((st = apply_dispatch(h, y, x, 0))) break;
         if (hipEventRecord(e0, c.stream) != hipSuccess) { st = RP_ERR_RUNTIME; break; }
         for (int r = 0; r < 3 && !st; r++) st = apply_dispatch(h, y, x, 0);
@@ -452,7 +551,7 @@ static int autotune(rp_terms *h, const uint64_t *bits, const double *coeff, uint
             if (!st) st = set_error(RP_ERR_RUNTIME, "rp_terms_upload_ex: autotuning timing failed");
             break;
         }
-        if (ms < best) { best = ms; btb = cd[0]; bnt = cd[1]; }
+        if (ms < best) { best = ms; pick = cd; }
     }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
@@ -460,12 +559,8 @@ static int autotune(rp_terms *h, const uint64_t *bits, const double *coeff, uint
     (void)hipFree(x);
     (void)hipFree(y);
     if (st) return st;
-    if (btb != h->tile_bits || bnt != h->threads) {
-        free_tables(h);
-        h->tile_bits = btb;
-        h->threads = bnt;
-        st = upload_any(h, bits, coeff, n);
-    }
+    if (pick.tb != h->tile_bits || pick.nt != h->threads || pick.depth != h->depth)
+        st = retile(h, pick.tb, pick.nt, pick.depth);
     return st;
 }
 
@@ -487,9 +582,11 @@ int rp_terms_upload_ex(rp_terms_t *out, int prec, int len, const uint64_t *bits,
     h->len = len;
     h->acc64 = (prec == RP_C128 || !(flags & RP_TERMS_ACC_NATIVE)) ? 1 : 0;
     choose_tiling(prec, len, &h->tile_bits, &h->threads);
+    h->raw_bits.assign(bits, bits + nterms);
+    h->raw_coeff.assign(coeff, coeff + 2 * nterms);
     st = upload_any(h, bits, coeff, nterms);
     if (!st && (flags & RP_TERMS_AUTOTUNE) && h->tile_bits && h->nterms && len <= 31)
-        st = autotune(h, bits, coeff, nterms);
+        st = autotune(h);
     if (st) {
         free_handle(h);
         return st;
@@ -514,12 +611,26 @@ int rp_terms_free(rp_terms_t h)
     return RP_OK;
 }
 
+int rp_terms_set_tiling(rp_terms_t h, int tile_bits, int threads, int depth)
+{
+    int st = check_init();
+    if (st) return st;
+    RP_REQUIRE(h, "null term handle");
+    RP_REQUIRE(tile_bits <= h->len, "tile bits exceed the chain length");
+    RP_REQUIRE(tiling_instantiated(h->prec, tile_bits, threads, depth),
+               "tiling not instantiated for this precision");
+    RP_CHECK_HIP(hipStreamSynchronize(ctx().stream));
+    return retile(h, tile_bits, threads, depth);
+}
+
 int rp_terms_info(rp_terms_t h, rp_terms_info_t *info)
 {
     RP_REQUIRE(h && info, "null argument");
     info->prec = h->prec;
     info->len = h->len;
     info->tile_bits = h->tile_bits;
+    info->threads = h->threads;
+    info->depth = h->depth;
     info->nterms = h->nterms;
     info->nflips = h->nflips;
     info->ngroups = h->tile_bits ? h->ngroups : h->nflips;

@@ -105,5 +105,44 @@ def test_autotuned_upload_matches_oracle(gpu, orc, fp, L):
     ref = orc.apply(L, bits, coeff, psi, fp=64)
     got = s.store().astype(np.complex128)
     print("autotuned L=%d fp%d: tile bits %d" % (L, fp, info.tile_bits), flush=True)
-    assert info.tile_bits in (8, 10, 12, 13)
+    assert (info.tile_bits, info.threads, info.depth) in [
+        (8, 128, 2), (10, 128, 2), (11, 256, 4), (12, 512, 2), (12, 512, 3), (12, 512, 4),
+        (13, 1024, 2)]
     assert np.abs(got - ref).max() < (1e-5 if fp == 32 else 1e-12) * np.abs(ref).max()
+
+
+TILINGS = {32: [(8, 128, 2), (10, 128, 2), (12, 512, 2), (12, 512, 3), (12, 512, 4),
+                (13, 1024, 2)],
+           64: [(8, 128, 2), (10, 128, 2), (11, 256, 4), (12, 512, 2)]}
+
+
+@pytest.mark.parametrize("fp", [32, 64])
+def test_every_tiling_matches_oracle(gpu, orc, fp):
+    """Every instantiated (tile bits, threads, pipeline depth) of the tiled
+    kernel, set through rp_terms_set_tiling, on SYK (real/imaginary subs,
+    many groups) and a general sum (complex and multi-term subs)."""
+    import ctypes
+    from gpu_util import DeviceBuffer, Terms, prec_of
+    rng = np.random.default_rng(fp)
+    dt = np.complex64 if fp == 32 else np.complex128
+    cases = [(16, *orc.syk_terms(32, 8.0, seed=17, fp=fp))]
+    b, c = random_pauli_sum(rng, 15, 60)
+    cases.append((15, np.concatenate([b, b[:7]]), np.concatenate([c, rng.standard_normal(7)])))
+    for L, bits, coeff in cases:
+        psi = random_state(rng, L, dt)
+        ref = orc.apply(L, bits, coeff, psi.astype(np.complex128), fp=64)
+        t = Terms(L, bits, coeff, prec_of(dt))
+        vin = DeviceBuffer.from_array(psi)
+        vout = DeviceBuffer(psi.nbytes)
+        for tb, nt, d in TILINGS[fp]:
+            gpu.check(gpu.hip().rp_terms_set_tiling(t.h, tb, nt, d))
+            info = t.info()
+            assert (info.tile_bits, info.threads, info.depth) == (tb, nt, d)
+            gpu.check(gpu.hip().rp_apply_ops(t.h, vout.ptr, vin.ptr, 0))
+            got = vout.to_array(dt, len(psi)).astype(np.complex128)
+            err = np.abs(got - ref).max() / np.abs(ref).max()
+            print("L=%d fp%d tiling %s: %.2e" % (L, fp, (tb, nt, d), err), flush=True)
+            assert err < (1e-6 if fp == 32 else 1e-13)
+        # a tiling that is not instantiated is an error, not a launch
+        assert gpu.hip().rp_terms_set_tiling(t.h, 9, 128, 2) != 0
+        vin.free(); vout.free(); t.free()
