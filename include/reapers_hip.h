@@ -122,6 +122,11 @@ int rp_terms_upload(rp_terms_t *out, int prec, int len, const uint64_t *bits,
 /* RP_TERMS_AUTOTUNE times the instantiated apply tilings (tile bits and
  * workgroup size) on scratch vectors at upload and keeps the fastest */
 #define RP_TERMS_AUTOTUNE 2
+/* RP_TERMS_ACC_GROUP (complex<float> only): the terms of one tile group are
+ * summed in FP32 and the group sums in FP64 -- FP32 arithmetic for the
+ * inner loop, one FP32 rounding per group instead of one per term in the
+ * running sum (used by the 2-stage pipelines; others accumulate in FP64) */
+#define RP_TERMS_ACC_GROUP 4
 int rp_terms_upload_ex(rp_terms_t *out, int prec, int len, const uint64_t *bits,
                        const double *coeff, uint64_t nterms, int flags);
 /* DevSumOps::release (inc/gpuimpl.h:1087-1105) */
@@ -218,7 +223,10 @@ int rp_batch_apply(rp_batch_t b, void *res, const void *vec, int accumulate);
 int rp_batch_normalize(rp_batch_t b, int i, double *nrm_dev);
 /* Arnoldi step j: V_{j+1} = A V_j, two Gram-Schmidt passes (FP64 dots),
  * H(0..j+1, j) into H_dev (per state a column-major (kmax+1) x kmax complex
- * double matrix, states consecutive), V_{j+1} normalized; kernels only */
+ * double matrix, states consecutive), V_{j+1} normalized; kernels only.
+ * H(j+1, j) is real: its real part is the norm after both passes, its
+ * imaginary slot holds the norm after the first pass (for the reference's
+ * breakdown test, norm < norm_1 / sqrt(2): inc/algo.h:79-106). */
 int rp_batch_arnoldi_step(rp_batch_t b, int j, double *H_dev);
 /* y += sum_{i<k} c[rhs][i] V_i, c complex double (device), cstride doubles per state */
 int rp_batch_lincomb(rp_batch_t b, void *y, int k, const double *c_dev, int cstride);

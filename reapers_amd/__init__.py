@@ -22,7 +22,7 @@ RP_C64, RP_C128 = 0, 1
 RP_ROUND_NEAREST, RP_ROUND_TOWARD_ZERO = 0, 1
 RP_H2D, RP_D2H, RP_D2D = 0, 1, 2
 RP_KERNEL_APPLY, RP_KERNEL_GS, RP_KERNEL_OTHER = 0, 1, 2
-RP_TERMS_ACC_NATIVE, RP_TERMS_AUTOTUNE = 1, 2
+RP_TERMS_ACC_NATIVE, RP_TERMS_AUTOTUNE, RP_TERMS_ACC_GROUP = 1, 2, 4
 
 _c = ctypes
 _vp = _c.c_void_p
@@ -86,6 +86,7 @@ def hip():
             "rp_vec_norm": [_c.c_int, _vp, _u64, _c.POINTER(_dbl)],
             "rp_init_random": [_c.c_int, _vp, _u64, _u64],
             "rp_terms_upload": [_c.POINTER(_vp), _c.c_int, _c.c_int, _vp, _vp, _u64],
+            "rp_terms_upload_ex": [_c.POINTER(_vp), _c.c_int, _c.c_int, _vp, _vp, _u64, _c.c_int],
             "rp_terms_free": [_vp],
             "rp_terms_info": [_vp, _c.POINTER(TermsInfo)],
             "rp_terms_set_tiling": [_vp, _c.c_int, _c.c_int, _c.c_int],

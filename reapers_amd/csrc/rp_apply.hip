@@ -52,6 +52,7 @@ struct rp_terms {
     int threads = 0;         // workgroup size of the tiled kernel
     int depth = 2;           // LDS tiles in the tiled kernel's pipeline
     int acc64 = 1;           // accumulate in FP64
+    int group_acc = 0;       // FP32 within a group, FP64 across groups (complex<float>)
     uint64_t nterms = 0, nflips = 0, ngroups = 0, nsubs = 0;
     void *d_groups = nullptr, *d_subs = nullptr, *d_terms = nullptr, *d_flat = nullptr;
     std::vector<uint64_t> raw_bits;   // the uploaded term list, for re-tiling
@@ -126,12 +127,12 @@ __device__ inline void tile_dma(const T *__restrict__ in, uint64_t src, T *dst)
 
 // Apply every sub of group G from the staged input tile `buf` (LDS) to the
 // RP accumulated rows of this thread.
-template <typename R, typename A, int TB, int NT>
+template <typename R, typename A, int TB, int NT, typename P = A>
 __device__ __forceinline__ void apply_group(const GroupRuns &G, const typename CxA<R>::T *buf,
                                             const SubRec<A> *__restrict__ subs,
                                             const TermRec<A> *__restrict__ terms, uint32_t t0,
-                                            uint32_t tid, A (&accr)[(1 << TB) / NT],
-                                            A (&acci)[(1 << TB) / NT])
+                                            uint32_t tid, P (&accr)[(1 << TB) / NT],
+                                            P (&acci)[(1 << TB) / NT])
 {
     using T = typename CxA<R>::T;
     constexpr int RP = (1 << TB) / NT;
@@ -147,58 +148,58 @@ __device__ __forceinline__ void apply_group(const GroupRuns &G, const typename C
     // real c, no phase bit among the row bits
     for (uint32_t s = G.b[kRunReal0]; s < G.b[kRunReal0 + 1]; s++) {
         RP_SUB_PROLOGUE
-        const A c = lneg ? -S.cr : S.cr;
+        const P c = (P)(lneg ? -S.cr : S.cr);
 #pragma unroll
         for (int r = 0; r < RP; r++) {
             const T v = row[(r ^ xr) * NT];
-            accr[r] += c * (A)v.x;
-            acci[r] += c * (A)v.y;
+            accr[r] += c * (P)v.x;
+            acci[r] += c * (P)v.y;
         }
     }
     for (uint32_t s = G.b[kRunReal1]; s < G.b[kRunReal1 + 1]; s++) {
         RP_SUB_PROLOGUE
-        const A c = lneg ? -S.cr : S.cr;
+        const P c = (P)(lneg ? -S.cr : S.cr);
 #pragma unroll
         for (int r = 0; r < RP; r++) {
             const T v = row[(r ^ xr) * NT];
-            const A cc = parity32(r & zr) ? -c : c;
-            accr[r] += cc * (A)v.x;
-            acci[r] += cc * (A)v.y;
+            const P cc = parity32(r & zr) ? -c : c;
+            accr[r] += cc * (P)v.x;
+            acci[r] += cc * (P)v.y;
         }
     }
     // imaginary c = i ci: (i ci)(vr + i vi) = -ci vi + i ci vr
     for (uint32_t s = G.b[kRunImag0]; s < G.b[kRunImag0 + 1]; s++) {
         RP_SUB_PROLOGUE
-        const A c = lneg ? -S.ci : S.ci;
+        const P c = (P)(lneg ? -S.ci : S.ci);
 #pragma unroll
         for (int r = 0; r < RP; r++) {
             const T v = row[(r ^ xr) * NT];
-            accr[r] -= c * (A)v.y;
-            acci[r] += c * (A)v.x;
+            accr[r] -= c * (P)v.y;
+            acci[r] += c * (P)v.x;
         }
     }
     for (uint32_t s = G.b[kRunImag1]; s < G.b[kRunImag1 + 1]; s++) {
         RP_SUB_PROLOGUE
-        const A c = lneg ? -S.ci : S.ci;
+        const P c = (P)(lneg ? -S.ci : S.ci);
 #pragma unroll
         for (int r = 0; r < RP; r++) {
             const T v = row[(r ^ xr) * NT];
-            const A cc = parity32(r & zr) ? -c : c;
-            accr[r] -= cc * (A)v.y;
-            acci[r] += cc * (A)v.x;
+            const P cc = parity32(r & zr) ? -c : c;
+            accr[r] -= cc * (P)v.y;
+            acci[r] += cc * (P)v.x;
         }
     }
     for (uint32_t s = G.b[kRunCplx]; s < G.b[kRunCplx + 1]; s++) {
         RP_SUB_PROLOGUE
-        const A cr = lneg ? -S.cr : S.cr;
-        const A ci = lneg ? -S.ci : S.ci;
+        const P cr = (P)(lneg ? -S.cr : S.cr);
+        const P ci = (P)(lneg ? -S.ci : S.ci);
 #pragma unroll
         for (int r = 0; r < RP; r++) {
             const T v = row[(r ^ xr) * NT];
             const bool n = parity32(r & zr);
-            const A a = n ? -cr : cr, c = n ? -ci : ci;
-            accr[r] += a * (A)v.x - c * (A)v.y;
-            acci[r] += a * (A)v.y + c * (A)v.x;
+            const P a = n ? -cr : cr, c = n ? -ci : ci;
+            accr[r] += a * (P)v.x - c * (P)v.y;
+            acci[r] += a * (P)v.y + c * (P)v.x;
         }
     }
     // several terms share x: d(m) = sum_t c_t (-1)^popcount(m & z_t),
@@ -210,14 +211,14 @@ __device__ __forceinline__ void apply_group(const GroupRuns &G, const typename C
         constexpr int RC = RP < 4 ? RP : 4;
 #pragma unroll
         for (int r0 = 0; r0 < RP; r0 += RC) {
-            A dr[RC], di[RC];
+            P dr[RC], di[RC];
 #pragma unroll
             for (int q = 0; q < RC; q++) { dr[q] = 0; di[q] = 0; }
             for (uint32_t t = S.term_begin; t < S.term_end; t++) {
                 const TermRec<A> C = terms[t];
                 const bool lneg = parity32((t0 | tid) & C.z);
-                const A cr = lneg ? -C.cr : C.cr;
-                const A ci = lneg ? -C.ci : C.ci;
+                const P cr = (P)(lneg ? -C.cr : C.cr);
+                const P ci = (P)(lneg ? -C.ci : C.ci);
                 const uint32_t zr = (C.z >> LB) & (RP - 1);
 #pragma unroll
                 for (int q = 0; q < RC; q++) {
@@ -229,8 +230,8 @@ __device__ __forceinline__ void apply_group(const GroupRuns &G, const typename C
 #pragma unroll
             for (int q = 0; q < RC; q++) {
                 const T v = row[((r0 + q) ^ xr) * NT];
-                accr[r0 + q] += dr[q] * (A)v.x - di[q] * (A)v.y;
-                acci[r0 + q] += dr[q] * (A)v.y + di[q] * (A)v.x;
+                accr[r0 + q] += dr[q] * (P)v.x - di[q] * (P)v.y;
+                acci[r0 + q] += dr[q] * (P)v.y + di[q] * (P)v.x;
             }
         }
     }
@@ -264,7 +265,7 @@ __device__ inline void tile_barrier()
 // flight while a group is applied.  Phase K of an unrolled round uses the
 // statically named buffer K, so the compiler sees which DMA writes which
 // array and does not drain every load before each LDS read.
-template <typename R, typename A, int TB, int NT, int D, int K>
+template <typename R, typename A, int TB, int NT, int D, int K, bool GA>
 __device__ __forceinline__ void pipe_phase(
     int g, int ngroups, const typename CxA<R>::T *__restrict__ in,
     const GroupRuns *__restrict__ groups, const SubRec<A> *__restrict__ subs,
@@ -288,10 +289,21 @@ __device__ __forceinline__ void pipe_phase(
     }
     if (g + D - 1 < ngroups)
         tile_dma<T, TB, NT>(in, t0 ^ (groups[g + D - 1].xh << TB), bufs[(K + D - 1) % D]);
-    apply_group<R, A, TB, NT>(groups[g], bufs[K], subs, terms, t0, tid, accr, acci);
+    if constexpr (GA) {
+        // the group's terms summed in FP32, the group sums added in FP64
+        constexpr int RP = (1 << TB) / NT;
+        float gr[RP], gi[RP];
+#pragma unroll
+        for (int r = 0; r < RP; r++) { gr[r] = 0; gi[r] = 0; }
+        apply_group<R, A, TB, NT, float>(groups[g], bufs[K], subs, terms, t0, tid, gr, gi);
+#pragma unroll
+        for (int r = 0; r < RP; r++) { accr[r] += (A)gr[r]; acci[r] += (A)gi[r]; }
+    } else {
+        apply_group<R, A, TB, NT>(groups[g], bufs[K], subs, terms, t0, tid, accr, acci);
+    }
 }
 
-template <typename R, typename A, int TB, int NT, int D>
+template <typename R, typename A, int TB, int NT, int D, bool GA>
 __global__ void __launch_bounds__(NT, (apply_waves_per_eu<R, TB, NT, D>()))
 k_apply_tiled(const typename CxA<R>::T *__restrict__ in, typename CxA<R>::T *__restrict__ out,
               const GroupRuns *__restrict__ groups, int ngroups, const SubRec<A> *__restrict__ subs,
@@ -325,19 +337,19 @@ k_apply_tiled(const typename CxA<R>::T *__restrict__ in, typename CxA<R>::T *__r
     for (int q = 0; q < D - 1; q++)
         if (q < ngroups) tile_dma<T, TB, NT>(in, t0 ^ (groups[q].xh << TB), ring[q]);
     for (int g = 0; g < ngroups; g += D) {
-        pipe_phase<R, A, TB, NT, D, 0>(g, ngroups, in, groups, subs, terms, t0, tid, tile0, tile1,
+        pipe_phase<R, A, TB, NT, D, 0, GA>(g, ngroups, in, groups, subs, terms, t0, tid, tile0, tile1,
                                        tile2, tile3, accr, acci);
         if (g + 1 >= ngroups) break;
-        pipe_phase<R, A, TB, NT, D, 1>(g + 1, ngroups, in, groups, subs, terms, t0, tid, tile0,
+        pipe_phase<R, A, TB, NT, D, 1, GA>(g + 1, ngroups, in, groups, subs, terms, t0, tid, tile0,
                                        tile1, tile2, tile3, accr, acci);
         if constexpr (D > 2) {
             if (g + 2 >= ngroups) break;
-            pipe_phase<R, A, TB, NT, D, 2>(g + 2, ngroups, in, groups, subs, terms, t0, tid, tile0,
+            pipe_phase<R, A, TB, NT, D, 2, GA>(g + 2, ngroups, in, groups, subs, terms, t0, tid, tile0,
                                            tile1, tile2, tile3, accr, acci);
         }
         if constexpr (D > 3) {
             if (g + 3 >= ngroups) break;
-            pipe_phase<R, A, TB, NT, D, 3>(g + 3, ngroups, in, groups, subs, terms, t0, tid, tile0,
+            pipe_phase<R, A, TB, NT, D, 3, GA>(g + 3, ngroups, in, groups, subs, terms, t0, tid, tile0,
                                            tile1, tile2, tile3, accr, acci);
         }
     }
@@ -375,12 +387,15 @@ static bool tiling_instantiated(int prec, int tb, int nt, int d)
     return false;
 }
 
-// tile bits b and workgroup size: 8 amplitudes per thread, two 64 KiB LDS
-// tiles (2^13 complex<float> or 2^12 complex<double>) at full size, smaller
-// tiles and workgroups for short chains so that there are enough workgroups
+// default tile bits b and workgroup size (RP_TERMS_AUTOTUNE measures the
+// alternatives): 8 amplitudes per thread, smaller tiles and workgroups for
+// short chains so that there are enough workgroups
 static void choose_tiling(int prec, int len, int *tb, int *nt)
 {
-    if (prec == RP_C64 && len >= 20) { *tb = 13; *nt = 1024; return; }
+    // 2^12-amplitude tiles with 512 threads: for complex<float> sparse SYK at
+    // 24 qubits faster than 2^13 tiles (more groups, but two workgroups per
+    // CU instead of one); for complex<double> the largest tile in 128 KiB
+    (void)prec;
     if (len >= 18) { *tb = 12; *nt = 512; return; }
     if (len >= 14) { *tb = 10; *nt = 128; return; }
     if (len >= 10) { *tb = 8; *nt = 128; return; }
@@ -441,9 +456,19 @@ static int launch_tiled(rp_terms *h, void *res, const void *vec, int accumulate)
 {
     using T = typename CxA<R>::T;
     const uint64_t grid = (1ull << h->len) >> TB;
-    RP_LAUNCH(RP_KERNEL_APPLY, (k_apply_tiled<R, A, TB, NT, D>), dim3((unsigned)grid), dim3(NT), 0,
-              (const T *)vec, (T *)res, (const GroupRuns *)h->d_groups, (int)h->ngroups,
-              (const SubRec<A> *)h->d_subs, (const TermRec<A> *)h->d_terms, accumulate);
+    if constexpr (sizeof(R) == 4 && sizeof(A) == 8 && D == 2) {
+        if (h->group_acc) {
+            RP_LAUNCH(RP_KERNEL_APPLY, (k_apply_tiled<R, A, TB, NT, D, true>), dim3((unsigned)grid),
+                      dim3(NT), 0, (const T *)vec, (T *)res, (const GroupRuns *)h->d_groups,
+                      (int)h->ngroups, (const SubRec<A> *)h->d_subs,
+                      (const TermRec<A> *)h->d_terms, accumulate);
+            return RP_OK;
+        }
+    }
+    RP_LAUNCH(RP_KERNEL_APPLY, (k_apply_tiled<R, A, TB, NT, D, false>), dim3((unsigned)grid),
+              dim3(NT), 0, (const T *)vec, (T *)res, (const GroupRuns *)h->d_groups,
+              (int)h->ngroups, (const SubRec<A> *)h->d_subs, (const TermRec<A> *)h->d_terms,
+              accumulate);
     return RP_OK;
 }
 
@@ -576,11 +601,15 @@ int rp_terms_upload_ex(rp_terms_t *out, int prec, int len, const uint64_t *bits,
     RP_REQUIRE(valid_prec(prec), "bad precision");
     RP_REQUIRE(len >= 1 && len <= 32, "len must be in [1, 32]");
     RP_REQUIRE(nterms == 0 || (bits && coeff), "null term arrays");
-    RP_REQUIRE((flags & ~(RP_TERMS_ACC_NATIVE | RP_TERMS_AUTOTUNE)) == 0, "unknown flags");
+    RP_REQUIRE((flags & ~(RP_TERMS_ACC_NATIVE | RP_TERMS_AUTOTUNE | RP_TERMS_ACC_GROUP)) == 0,
+               "unknown flags");
+    RP_REQUIRE(!((flags & RP_TERMS_ACC_NATIVE) && (flags & RP_TERMS_ACC_GROUP)),
+               "RP_TERMS_ACC_NATIVE and RP_TERMS_ACC_GROUP exclude each other");
     rp_terms *h = new rp_terms;
     h->prec = prec;
     h->len = len;
     h->acc64 = (prec == RP_C128 || !(flags & RP_TERMS_ACC_NATIVE)) ? 1 : 0;
+    h->group_acc = (prec == RP_C64 && (flags & RP_TERMS_ACC_GROUP)) ? 1 : 0;
     choose_tiling(prec, len, &h->tile_bits, &h->threads);
     h->raw_bits.assign(bits, bits + nterms);
     h->raw_coeff.assign(coeff, coeff + 2 * nterms);

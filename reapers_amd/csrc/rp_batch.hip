@@ -423,11 +423,13 @@ k_batch_norm_part(const typename CxA<R>::T *__restrict__ v, const ProbDesc *__re
 
 // per rhs: nrm = sqrt(sum of the problem's chunk partials); inv = 1/nrm
 // (0 if nrm == 0); optionally H(j+1, j) = nrm and nrm_out[rhs] = nrm
+// im_only: write the norm to Im H(j+1, j) only (the norm after the first
+// Gram-Schmidt pass, kept for the breakdown test) and leave inv alone
 __global__ void k_batch_norm_final(const double *__restrict__ npart,
                                    const ProbDesc *__restrict__ probs,
                                    const int *__restrict__ active, int nprob, int ns,
                                    double *__restrict__ inv, double *__restrict__ nrm_out,
-                                   double *__restrict__ H, int j, int kmax)
+                                   double *__restrict__ H, int j, int kmax, int im_only)
 {
     const int rhs = blockIdx.x * blockDim.x + threadIdx.x;
     if (rhs >= nprob * ns) return;
@@ -437,13 +439,15 @@ __global__ void k_batch_norm_final(const double *__restrict__ npart,
     double a = 0;
     for (uint32_t c = P.chunk0; c < P.chunk0 + P.nchunks; c++) a += npart[(size_t)c * ns + s];
     const double n = sqrt(a);
+    double *Hij = H ? H + (size_t)rhs * (2 * (kmax + 1) * kmax) + 2 * ((size_t)j * (kmax + 1) + j + 1)
+                    : nullptr;
+    if (im_only) {
+        if (Hij) Hij[1] = n;
+        return;
+    }
     inv[rhs] = n > 0 ? 1.0 / n : 0.0;
     if (nrm_out) nrm_out[rhs] = n;
-    if (H) {
-        double *Hij = H + (size_t)rhs * (2 * (kmax + 1) * kmax) + 2 * ((size_t)j * (kmax + 1) + j + 1);
-        Hij[0] = n;
-        Hij[1] = 0;
-    }
+    if (Hij) Hij[0] = n;
 }
 
 // v *= inv[rhs]
@@ -899,7 +903,7 @@ int rp_batch_normalize(rp_batch_t b, int i, double *nrm_out)
     }
     RP_LAUNCH(RP_KERNEL_GS, k_batch_norm_final, dim3((nr + 255) / 256), dim3(256), 0,
               (const double *)b->d_npart, P, act, b->nprob, b->ns, (double *)b->d_inv, nrm_out,
-              (double *)nullptr, 0, b->kmax);
+              (double *)nullptr, 0, b->kmax, 0);
     if (b->prec == RP_C64) {
         RP_LAUNCH(RP_KERNEL_GS, k_batch_scale<float>, gc, dim3(kRedT), 0, (float2 *)b->V[i], P,
                   cp, act, b->ns, b->chunk_bits, (const double *)b->d_inv);
@@ -950,10 +954,12 @@ int rp_batch_arnoldi_step(rp_batch_t b, int j, double *H)
                       (double2 *)b->V[j + 1], P, cp, act, b->ns, b->chunk_bits,
                       (const double *)b->d_h, hstride, (double *)b->d_npart);
         }
+        // after the first pass: its norm to Im H(j+1, j); after the second:
+        // the norm to Re H(j+1, j) and its inverse for the normalization
+        RP_LAUNCH(RP_KERNEL_GS, k_batch_norm_final, dim3((nr + 255) / 256), dim3(256), 0,
+                  (const double *)b->d_npart, P, act, b->nprob, b->ns, (double *)b->d_inv,
+                  (double *)nullptr, H, j, b->kmax, pass == 0 ? 1 : 0);
     }
-    RP_LAUNCH(RP_KERNEL_GS, k_batch_norm_final, dim3((nr + 255) / 256), dim3(256), 0,
-              (const double *)b->d_npart, P, act, b->nprob, b->ns, (double *)b->d_inv,
-              (double *)nullptr, H, j, b->kmax);
     if (b->prec == RP_C64) {
         RP_LAUNCH(RP_KERNEL_GS, k_batch_scale<float>, gc, dim3(kRedT), 0, (float2 *)b->V[j + 1],
                   P, cp, act, b->ns, b->chunk_bits, (const double *)b->d_inv);

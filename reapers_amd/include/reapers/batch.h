@@ -252,20 +252,36 @@ private:
             std::fill(ch.begin(), ch.end(), 0.0);
             std::vector<char> now_done(done);
             std::vector<double> nrm(nr, 0.0);
+            const double eta = 1 / std::sqrt(2.0);
 #pragma omp parallel for schedule(dynamic)
             for (int r = 0; r < nr; r++) {
                 if (done[r]) continue;
+                // Arnoldi breakdown at step j (the reference's test after its
+                // passes, inc/algo.h:79-106): the second pass left less than
+                // eta of the first pass's norm, or nothing.  The Krylov space
+                // is then invariant, the block is cut to j + 1 columns and
+                // the answer is exact (inc/algo.h:356).
+                int mr = m;
+                bool brk = false;
+                for (int j = 0; j < m && !brk; j++) {
+                    cd s = Hget(r, j + 1, j);
+                    if (s.real() == 0.0 || s.real() < eta * s.imag()) {
+                        brk = true;
+                        mr = j + 1;
+                    }
+                }
                 dense::CMat &F = Hfull[r];
-                F.grow(off_blk + m);
+                F.grow(off_blk + mr);
                 if (off_blk) F(off_blk, off_blk - 1) = oldbeta[r];
-                for (int i = 0; i < m; i++)
-                    for (int j = 0; j < m; j++)
-                        F(off_blk + i, off_blk + j) = i <= j + 1 ? Hget(r, i, j) : 0.0;
+                for (int i = 0; i < mr; i++)
+                    for (int j = 0; j < mr; j++)
+                        F(off_blk + i, off_blk + j) =
+                            i == j + 1 ? cd(Hget(r, i, j).real()) : i <= j ? Hget(r, i, j) : 0.0;
                 dense::CMat tH = F;
                 for (auto &v : tH.a) v *= z;
                 dense::CMat E = dense::expm(tH);
                 double nrm2 = 0;
-                for (int i = 0; i < m; i++) {
+                for (int i = 0; i < mr; i++) {
                     cd e = E(off_blk + i, 0);
                     cd c = beta0[r] * e;
                     ch[(size_t)2 * kmax * r + 2 * i] = c.real();
@@ -274,11 +290,6 @@ private:
                 }
                 nrm[r] = std::sqrt(nrm2);
                 oldbeta[r] = Hget(r, m, m - 1).real();
-                // a zero norm (inverse 0 on the device) is the Arnoldi
-                // breakdown: the Krylov space is invariant and the answer
-                // exact (inc/algo.h:356)
-                bool brk = false;
-                for (int j = 0; j < m; j++) brk = brk || Hget(r, j + 1, j) == 0.0;
                 if (brk || nrm[r] < tol || (max_iters && niters >= max_iters)) now_done[r] = 1;
             }
             st.host_ms += std::chrono::duration<double, std::milli>(

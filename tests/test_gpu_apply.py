@@ -146,3 +146,33 @@ def test_every_tiling_matches_oracle(gpu, orc, fp):
         # a tiling that is not instantiated is an error, not a launch
         assert gpu.hip().rp_terms_set_tiling(t.h, 9, 128, 2) != 0
         vin.free(); vout.free(); t.free()
+
+
+@pytest.mark.parametrize("tiling", [(12, 512), (13, 1024), (10, 128)])
+def test_group_accumulation_matches_oracle(gpu, orc, tiling):
+    """RP_TERMS_ACC_GROUP (FP32 sums within a tile group, FP64 across
+    groups) against the FP64 oracle: as accurate as the FP32 input allows."""
+    from gpu_util import DeviceBuffer, prec_of
+    import ctypes
+    L = 18
+    rng = np.random.default_rng(sum(tiling))
+    bits, coeff = orc.syk_terms(36, 8.0, seed=3, fp=32)
+    psi = random_state(rng, L, np.complex64)
+    ref = orc.apply(L, bits, coeff, psi.astype(np.complex128), fp=64)
+    b = np.ascontiguousarray(bits, np.uint64)
+    co = np.empty(2 * len(b))
+    co[0::2], co[1::2] = np.real(coeff), np.imag(coeff)
+    h = ctypes.c_void_p()
+    gpu.check(gpu.hip().rp_terms_upload_ex(ctypes.byref(h), gpu.RP_C64, L,
+                                           b.ctypes.data_as(ctypes.c_void_p),
+                                           co.ctypes.data_as(ctypes.c_void_p), len(b),
+                                           gpu.RP_TERMS_ACC_GROUP))
+    gpu.check(gpu.hip().rp_terms_set_tiling(h, tiling[0], tiling[1], 2))
+    vin = DeviceBuffer.from_array(psi)
+    vout = DeviceBuffer(psi.nbytes)
+    gpu.check(gpu.hip().rp_apply_ops(h, vout.ptr, vin.ptr, 0))
+    got = vout.to_array(np.complex64, len(psi)).astype(np.complex128)
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    print("group accumulation %s: %.2e" % (tiling, err), flush=True)
+    assert err < 2e-7
+    gpu.check(gpu.hip().rp_terms_free(h))
