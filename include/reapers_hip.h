@@ -168,6 +168,15 @@ int rp_apply_ops(rp_terms_t h, void *res, const void *vec, int accumulate);
  * h receives 2k doubles (re, im).  k may be 0 (then only the norm). */
 int rp_gs_pass(int prec, const void *const *V, int k, void *w, uint64_t n,
                double *h, double *norm);
+/* rp_gs_pass with the normalization fused into the update sweep: when the
+ * coefficients give beta^2 = ||w||^2 - sum |h_q|^2 >= ||w||^2 / 2 (the
+ * Pythagorean norm of the result for an orthonormal basis, free of
+ * cancellation) and beta >= fuse_min > 0, w is written as (w - V h) / beta,
+ * *normalized = 1 and *norm = beta; otherwise the plain pass (*normalized =
+ * 0, *norm = the computed norm).  Saves the separate scale sweep of the
+ * Arnoldi step (inc/algo.h:170). */
+int rp_gs_pass_fused(int prec, const void *const *V, int k, void *w, uint64_t n, double *h,
+                     double *norm, double fuse_min, int *normalized);
 /* y += sum_i c[i] V[i] for k vectors, c given as 2k doubles (re, im) */
 int rp_lincomb(int prec, void *y, const void *const *V, int k, const double *c, uint64_t n);
 

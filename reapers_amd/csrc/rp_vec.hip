@@ -193,18 +193,19 @@ k_reduce_cols(const double *__restrict__ partial, int nparts, int width, double 
 }
 
 // <V[q]|w> for the KC vectors of one chunk, FP64 partials per block written
-// to partial[block*width + 2*(q0+q) + {0,1}].  All KC loads of an element are
-// issued before any use.
+// to partial[block*width + 2*(q0+q) + {0,1}]; with wnorm, also ||w||^2 to
+// partial[block*width + 2*kMaxVecs].  All KC loads of an element are issued
+// before any use.
 template <typename R, int KC>
 __global__ void __launch_bounds__(kRedThreads)
 k_dots(VecPtrs16 V, const typename Cx<R>::T *__restrict__ w, uint64_t n,
-       double *__restrict__ partial, int width, int q0)
+       double *__restrict__ partial, int width, int q0, int wnorm)
 {
     using T = typename Cx<R>::T;
-    __shared__ double sh[(kWaves + 1) * 2 * KC];
-    double acc[2 * KC];
+    __shared__ double sh[(kWaves + 1) * (2 * KC + 1)];
+    double acc[2 * KC + 1];
 #pragma unroll
-    for (int q = 0; q < 2 * KC; q++) acc[q] = 0.0;
+    for (int q = 0; q < 2 * KC + 1; q++) acc[q] = 0.0;
     GRID_STRIDE(i, n) {
         const T y = w[i];
         T x[KC];
@@ -217,25 +218,51 @@ k_dots(VecPtrs16 V, const typename Cx<R>::T *__restrict__ w, uint64_t n,
             acc[2 * q] += xr * yr + xi * yi;
             acc[2 * q + 1] += xr * yi - xi * yr;
         }
+        acc[2 * KC] += yr * yr + yi * yi;
     }
-    block_reduce<2 * KC>(acc, sh);
+    block_reduce<2 * KC + 1>(acc, sh);
     if (threadIdx.x < 2 * KC)
         partial[(size_t)blockIdx.x * width + 2 * q0 + threadIdx.x] =
-            sh[kWaves * 2 * KC + threadIdx.x];
+            sh[kWaves * (2 * KC + 1) + threadIdx.x];
+    if (wnorm && threadIdx.x == 2 * KC)
+        partial[(size_t)blockIdx.x * width + 2 * kMaxVecs] = sh[kWaves * (2 * KC + 1) + 2 * KC];
 }
 
 // w -= sum_q h[q0+q] V[q] for the KC vectors of one chunk (FP64 arithmetic,
-// one rounding); the last chunk also writes ||w||^2 partials
+// one rounding); the last chunk also writes ||w||^2 partials.
+// Fused normalization (last chunk, fuse_min > 0): with all k coefficients h
+// and ||w||^2 before the pass in res, beta^2 = ||w||^2 - sum |h_q|^2 is the
+// norm of the result when the basis is orthonormal; if beta^2 >= ||w||^2 / 2
+// (no cancellation to speak of) and beta >= fuse_min, the result is written
+// scaled by 1/beta and beta is stored to res[2 kMaxVecs + 2] (else 0).
 template <typename R, int KC>
 __global__ void __launch_bounds__(kRedThreads)
 k_gs_update(VecPtrs16 V, const double *__restrict__ h, typename Cx<R>::T *__restrict__ w,
-            uint64_t n, double *__restrict__ partial)
+            uint64_t n, double *__restrict__ partial, const double *__restrict__ res, int k,
+            double fuse_min)
 {
     using T = typename Cx<R>::T;
     __shared__ double hs[2 * KC];
     __shared__ double sh[(kWaves + 1) * 1];
+    __shared__ double fscale;
     if (threadIdx.x < 2 * KC) hs[threadIdx.x] = h[threadIdx.x];
+    if (threadIdx.x == 0) {
+        double scale = 1.0, beta = 0.0;
+        if (fuse_min > 0) {
+            double s2 = 0;
+            for (int q = 0; q < k; q++) s2 += res[2 * q] * res[2 * q] + res[2 * q + 1] * res[2 * q + 1];
+            const double wn2 = res[2 * kMaxVecs + 1];
+            const double b2 = wn2 - s2;
+            if (b2 > 0 && b2 >= 0.5 * wn2 && sqrt(b2) >= fuse_min) {
+                beta = sqrt(b2);
+                scale = 1.0 / beta;
+            }
+            if (blockIdx.x == 0) ((double *)res)[2 * kMaxVecs + 2] = beta;
+        }
+        fscale = scale;
+    }
     __syncthreads();
+    const double f = fscale;
     double nacc[1] = {0.0};
     GRID_STRIDE(i, n) {
         const T y = w[i];
@@ -250,8 +277,8 @@ k_gs_update(VecPtrs16 V, const double *__restrict__ h, typename Cx<R>::T *__rest
             yi -= hr * xi + hi * xr;
         }
         T o;
-        o.x = (R)yr;
-        o.y = (R)yi;
+        o.x = (R)(yr * f);
+        o.y = (R)(yi * f);
         w[i] = o;
         const double orr = o.x, oi = o.y;
         nacc[0] += orr * orr + oi * oi;
@@ -324,8 +351,10 @@ k_dot_partial(const typename Cx<R>::T *__restrict__ a, const typename Cx<R>::T *
 // host helpers
 // ---------------------------------------------------------------------------
 static inline double *ws_partial() { return (double *)ctx().work; }
-// [0, B*2*kMaxVecs): dot partials; [B*2*kMaxVecs, B*(2*kMaxVecs+1)): norm
-// partials; then the reduced results (B = kRedBlocks)
+// [0, B*(2*kMaxVecs+1)): dot partials (2 kMaxVecs columns of coefficients
+// and one of ||w||^2 per block); [B*2*kMaxVecs, B*(2*kMaxVecs+1)): norm
+// partials of the update, written after the dot partials are reduced; then
+// the reduced results (B = kRedBlocks)
 static inline double *ws_result()
 {
     return (double *)ctx().work + (size_t)kRedBlocks * (2 * kMaxVecs + 1);
@@ -385,31 +414,39 @@ static int nrm2_launch(const void *a, uint64_t n, double *dst_dev)
     case 15: LAUNCH(15); break; default: LAUNCH(16); break;                       \
     }
 
+// One Gram-Schmidt pass; res layout: [0, 2k) h, [2 kMaxVecs] ||w_out||^2,
+// [2 kMaxVecs + 1] ||w_in||^2, [2 kMaxVecs + 2] fused beta (0: not fused)
 template <typename R>
 static int gs_pass_impl(const void *const *V, int k, void *w, uint64_t n, double *h,
-                        double *norm)
+                        double *norm, double fuse_min, int *normalized)
 {
     using T = typename Cx<R>::T;
     const int nb = red_blocks(n);
-    const int width = 2 * kMaxVecs;
-    double *res = ws_result();                    // res[0..2k): h, res[2k]: ||w||^2
-    double *npart = ws_partial() + (size_t)kRedBlocks * width;   // norm partials
+    const int width = 2 * kMaxVecs + 1;
+    double *res = ws_result();
+    double *npart = ws_partial() + (size_t)kRedBlocks * (2 * kMaxVecs);   // norm partials
+    const bool fuse = fuse_min > 0 && k > 0;
     if (k > 0) {
         for (int q0 = 0; q0 < k; q0 += kDotChunk) {
             const int kc = k - q0 < kDotChunk ? k - q0 : kDotChunk;
             VecPtrs16 P{};
             for (int q = 0; q < kc; q++) P.p[q] = V[q0 + q];
+            const int wn = fuse && q0 == 0;
 #define RP_DOTS(KC)                                                               \
     RP_LAUNCH(RP_KERNEL_GS, (k_dots<R, KC>), dim3(nb), dim3(kRedThreads), 0, P, (const T *)w, n, \
-              ws_partial(), width, q0)
+              ws_partial(), width, q0, wn)
             RP_KC_SWITCH(kc, RP_DOTS)
 #undef RP_DOTS
         }
         RP_LAUNCH(RP_KERNEL_GS, k_reduce_cols, dim3(2 * k), dim3(kRedThreads), 0,
                   (const double *)ws_partial(), nb, width, res);
+        if (fuse)
+            RP_LAUNCH(RP_KERNEL_GS, k_reduce_cols, dim3(1), dim3(kRedThreads), 0,
+                      (const double *)ws_partial() + 2 * kMaxVecs, nb, width,
+                      res + 2 * kMaxVecs + 1);
     }
     if (k == 0) {
-        int st = nrm2_launch<R>(w, n, res);
+        int st = nrm2_launch<R>(w, n, res + 2 * kMaxVecs);
         if (st) return st;
     } else {
         for (int q0 = 0; q0 < k; q0 += kUpdChunk) {
@@ -419,18 +456,21 @@ static int gs_pass_impl(const void *const *V, int k, void *w, uint64_t n, double
             for (int q = 0; q < kc; q++) P.p[q] = V[q0 + q];
 #define RP_UPD(KC)                                                                \
     RP_LAUNCH(RP_KERNEL_GS, (k_gs_update<R, KC>), dim3(nb), dim3(kRedThreads), 0, P,          \
-              (const double *)(res + 2 * q0), (T *)w, n, last ? npart : (double *)nullptr)
+              (const double *)(res + 2 * q0), (T *)w, n, last ? npart : (double *)nullptr,   \
+              (const double *)res, k, (last && fuse) ? fuse_min : 0.0)
             RP_KC_SWITCH(kc, RP_UPD)
 #undef RP_UPD
         }
         RP_LAUNCH(RP_KERNEL_GS, k_reduce_cols, dim3(1), dim3(kRedThreads), 0,
-                  (const double *)npart, nb, 1, res + 2 * k);
+                  (const double *)npart, nb, 1, res + 2 * kMaxVecs);
     }
-    double tmp[2 * kMaxVecs + 1];
-    int st = fetch_results(tmp, 2 * k + 1);
+    double tmp[2 * kMaxVecs + 3];
+    int st = fetch_results(tmp, 2 * kMaxVecs + 3);
     if (st) return st;
     for (int q = 0; q < 2 * k; q++) h[q] = tmp[q];
-    *norm = sqrt(tmp[2 * k]);
+    const double beta = fuse ? tmp[2 * kMaxVecs + 2] : 0.0;
+    if (normalized) *normalized = beta > 0 ? 1 : 0;
+    *norm = beta > 0 ? beta : sqrt(tmp[2 * kMaxVecs]);
     return RP_OK;
 }
 
@@ -639,8 +679,28 @@ int rp_gs_pass(int prec, const void *const *V, int k, void *w, uint64_t n, doubl
         *norm = 0;
         return RP_OK;
     }
-    return prec == RP_C64 ? gs_pass_impl<float>(V, k, w, n, h, norm)
-                          : gs_pass_impl<double>(V, k, w, n, h, norm);
+    return prec == RP_C64 ? gs_pass_impl<float>(V, k, w, n, h, norm, 0.0, nullptr)
+                          : gs_pass_impl<double>(V, k, w, n, h, norm, 0.0, nullptr);
+}
+
+int rp_gs_pass_fused(int prec, const void *const *V, int k, void *w, uint64_t n, double *h,
+                     double *norm, double fuse_min, int *normalized)
+{
+    int st = check_init();
+    if (st) return st;
+    RP_REQUIRE(valid_prec(prec), "bad precision");
+    RP_REQUIRE(k >= 0 && k <= kMaxVecs, "k must be in [0, 64]");
+    RP_REQUIRE(norm && normalized && (k == 0 || (h && V)), "null argument");
+    RP_REQUIRE(w || n == 0, "null vector");
+    for (int q = 0; q < k; q++) RP_REQUIRE(V[q], "null basis vector");
+    *normalized = 0;
+    if (n == 0) {
+        for (int q = 0; q < 2 * k; q++) h[q] = 0;
+        *norm = 0;
+        return RP_OK;
+    }
+    return prec == RP_C64 ? gs_pass_impl<float>(V, k, w, n, h, norm, fuse_min, normalized)
+                          : gs_pass_impl<double>(V, k, w, n, h, norm, fuse_min, normalized);
 }
 
 int rp_lincomb(int prec, void *y, const void *const *V, int k, const double *c, uint64_t n)

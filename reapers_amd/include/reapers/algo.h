@@ -19,6 +19,10 @@
 //     (at most 3, eta = 1/sqrt(2): inc/algo.h:79-101), or, with
 //     reorth = Reorth::Always, every step makes exactly two passes.
 //   - Hfull, H, beta and the norms are FP64 for FP32 states as well.
+//   - the pass that ends the Gram-Schmidt iteration also normalizes the new
+//     vector when its norm follows from the coefficients without
+//     cancellation (||w - V h||^2 = ||w||^2 - ||h||^2 >= ||w||^2 / 2), which
+//     saves the separate scaling sweep.
 //   - the optional sub-step splits t into equal pieces, each a separate
 //     restarted evolution (the reference evolves the whole t at once).
 #pragma once
@@ -29,10 +33,12 @@ class Krylov {
     using SumOps = typename Impl::template SumOps<FpType>;
     using cd = std::complex<double>;
 
-    // iterated fused Gram-Schmidt of buf(j+1) against buf(0..j)
+    // iterated fused Gram-Schmidt of buf(j+1) against buf(0..j).  A pass
+    // that ends the iteration (its norm at least eta times the previous one)
+    // may normalize buf(j+1) in the same sweep; normalized reports it.
     template<RealScalar FpType>
     static bool orthogonalize(State<FpType, Impl> &s, dense::CMat &H, int j, double &beta,
-                              Reorth mode, KrylovStats &st) {
+                              Reorth mode, KrylovStats &st, bool &normalized) {
         const double eta = 1 / std::sqrt(2.0);
         const size_t dim = s.dim();
         std::vector<const typename Impl::template VecType<FpType> *> V(j + 1);
@@ -41,18 +47,24 @@ class Krylov {
         double old_norm = 1.0, norm = eta / 2;
         int niter = 0;
         const int max_passes = mode == Reorth::Always ? 2 : 3;
+        normalized = false;
         while (niter < max_passes && norm != 0 &&
                (mode == Reorth::Always || norm < eta * old_norm)) {
             niter++;
-            double nrm = Impl::gs_pass(dim, V, s.buf(j + 1), r);
+            // the norm this pass must reach to end the iteration
+            double fuse_min = 0;
+            if (mode == Reorth::Always) fuse_min = niter == max_passes ? 1e-300 : 0;
+            else fuse_min = eta * norm;
+            double nrm = Impl::gs_pass(dim, V, s.buf(j + 1), r, fuse_min, normalized);
             for (int i = 0; i <= j; i++) h[i] += r[i];
             old_norm = norm;
             norm = nrm;
             st.gs_passes++;
+            if (normalized) break;
         }
         beta = norm;
         for (int i = 0; i <= j; i++) H(i, j) = h[i];
-        return norm != 0 && norm >= eta * old_norm;
+        return norm != 0 && (normalized || norm >= eta * old_norm);
     }
 
     // Arnoldi steps j0 .. m-1 from the orthonormal buf(0..j0) (inc/algo.h:144-173;
@@ -65,13 +77,14 @@ class Krylov {
         for (int j = j0; j < m; j++) {
             Impl::apply_ops_overwrite(len, s.buf(j + 1), A, s.buf(j));
             st.matvecs++;
-            bool ok = orthogonalize(s, H, j, beta, mode, st);
+            bool normalized;
+            bool ok = orthogonalize(s, H, j, beta, mode, st, normalized);
             if (!ok) {
                 m = j + 1;
                 return false;
             }
             if (j != m - 1) H(j + 1, j) = beta;
-            Impl::scale_vec(dim, s.buf(j + 1), (FpType)(1.0 / beta));
+            if (!normalized) Impl::scale_vec(dim, s.buf(j + 1), (FpType)(1.0 / beta));
         }
         return true;
     }

@@ -270,12 +270,24 @@ public:
     template<RealScalar FpType>
     static double gs_pass(size_t size, const std::vector<const DeviceVec<FpType> *> &V,
                           BufType<FpType> w, std::vector<std::complex<double>> &h) {
+        bool normalized;
+        return gs_pass(size, V, w, h, 0.0, normalized);
+    }
+
+    // the same pass, writing w / ||w|| when the norm is known to be at least
+    // fuse_min and free of cancellation (normalized is then true)
+    template<RealScalar FpType>
+    static double gs_pass(size_t size, const std::vector<const DeviceVec<FpType> *> &V,
+                          BufType<FpType> w, std::vector<std::complex<double>> &h,
+                          double fuse_min, bool &normalized) {
         std::vector<const void *> p(V.size());
         for (size_t i = 0; i < V.size(); i++) p[i] = V[i]->ptr;
         h.resize(V.size());
         double nrm;
-        RP_CALL(rp_gs_pass(internal::rp_prec<FpType>(), p.data(), (int)V.size(), w.ptr, size,
-                           (double *)h.data(), &nrm));
+        int nz = 0;
+        RP_CALL(rp_gs_pass_fused(internal::rp_prec<FpType>(), p.data(), (int)V.size(), w.ptr,
+                                 size, (double *)h.data(), &nrm, fuse_min, &nz));
+        normalized = nz != 0;
         return nrm;
     }
 

@@ -124,3 +124,49 @@ def test_init_random_is_normalized_and_seeded(gpu):
     # Gaussian moments of the unnormalized components: mean 0, variance 1/(2n)
     comp = np.concatenate([a.real, a.imag]) * np.sqrt(2 * n)
     assert abs(comp.mean()) < 0.02 and abs(comp.var() - 1) < 0.03
+
+
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+@pytest.mark.parametrize("k", [1, 7, 17, 40])
+def test_gs_pass_fused_normalization(gpu, dtype, k):
+    """rp_gs_pass_fused: with an orthonormal basis and a w mostly outside its
+    span, the pass writes (w - V h) / beta with beta = sqrt(||w||^2 - |h|^2);
+    with fuse_min above beta, or with heavy cancellation, it is the plain pass."""
+    rng = np.random.default_rng(7 * k)
+    n = 65536 + 5
+    A = rng.standard_normal((n, k)) + 1j * rng.standard_normal((n, k))
+    Q, _ = np.linalg.qr(A)
+    V = [Q[:, i].astype(dtype) for i in range(k)]
+    p = prec_of(dtype)
+    dV = [DeviceBuffer.from_array(v) for v in V]
+    ptrs = (ctypes.c_void_p * k)(*[d.ptr.value for d in dV])
+    Vd = [v.astype(complex) for v in V]
+    for case in ("fused", "too_small", "cancellation"):
+        if case == "cancellation":
+            # 99% of w inside the span: beta^2 < ||w||^2 / 2
+            w = (sum(Vd[i] * rng.standard_normal() for i in range(k)) * 10
+                 + _rand(rng, n, dtype).astype(complex) * 0.1).astype(dtype)
+        else:
+            w = _rand(rng, n, dtype)
+        dw = DeviceBuffer.from_array(w)
+        hbuf = np.zeros(2 * k)
+        nrm, nz = ctypes.c_double(), ctypes.c_int()
+        wd = w.astype(complex)
+        href = np.array([np.vdot(v, wd) for v in Vd])
+        wref = wd - sum(href[i] * Vd[i] for i in range(k))
+        beta = np.linalg.norm(wref)
+        fuse_min = 1e-3 if case != "too_small" else 2 * beta
+        ra.check(gpu.hip().rp_gs_pass_fused(p, ptrs, k, dw.ptr, n,
+                                            hbuf.ctypes.data_as(ctypes.c_void_p),
+                                            ctypes.byref(nrm), fuse_min, ctypes.byref(nz)))
+        wgot = dw.to_array(dtype, n).astype(complex)
+        tol = 2e-6 if dtype == np.complex64 else 1e-12
+        if case == "fused":
+            assert nz.value == 1
+            assert abs(nrm.value - beta) <= tol * beta
+            assert np.abs(wgot - wref / beta).max() <= tol * np.abs(wref / beta).max() * 4
+        else:
+            assert nz.value == 0
+            assert abs(nrm.value - np.linalg.norm(wgot)) <= 1e-12 * nrm.value
+            assert np.abs(wgot - wref).max() <= tol * (1 + np.abs(href).sum())
+        dw.free()
