@@ -143,9 +143,9 @@ def test_gs_pass_fused_normalization(gpu, dtype, k):
     Vd = [v.astype(complex) for v in V]
     for case in ("fused", "too_small", "cancellation"):
         if case == "cancellation":
-            # 99% of w inside the span: beta^2 < ||w||^2 / 2
-            w = (sum(Vd[i] * rng.standard_normal() for i in range(k)) * 10
-                 + _rand(rng, n, dtype).astype(complex) * 0.1).astype(dtype)
+            # nearly all of w inside the span: beta^2 << ||w||^2 / 2
+            inside = sum(Vd[i] * (1 + abs(rng.standard_normal())) for i in range(k))
+            w = (inside * 10 * np.sqrt(n) + _rand(rng, n, dtype).astype(complex) * 0.1).astype(dtype)
         else:
             w = _rand(rng, n, dtype)
         dw = DeviceBuffer.from_array(w)
