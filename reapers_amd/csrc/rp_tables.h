@@ -26,7 +26,8 @@ template <> struct CxA<double> { using T = double2; };
 enum : uint32_t { kSubReal = 0, kSubImag = 1, kSubComplex = 2, kSubMulti = 3 };
 
 template <typename A> struct SubRec {
-    uint32_t xl, z, kind, term_begin, term_end, pad0, pad1, pad2;
+    uint32_t xl, z, kind, term_begin, term_end, pad0;
+    float fr, fi;            // cr, ci rounded to float (for FP32 partial sums)
     A cr, ci;
 };
 template <typename A> struct TermRec { uint32_t z, pad; A cr, ci; };
@@ -36,10 +37,15 @@ __device__ inline uint32_t parity32(uint32_t v) { return __popc(v) & 1u; }
 
 // Sub classes, in the order a group stores its subs: single-term real c with
 // no phase bit in the row bits, real with row bits, imaginary (same split),
-// complex, and several terms per flip mask.
+// complex, and several terms per flip mask.  Within a class the subs are
+// sorted by their row permutation xr = (x mod 2^b) / threads (< 8 for every
+// tiling), so that a kernel can walk class k, permutation q as the range
+// [b[k*kRowsMax + q], b[k*kRowsMax + q + 1]) with q a compile-time constant;
+// class k as a whole is [b[k*kRowsMax], b[(k+1)*kRowsMax]).
 enum : uint32_t { kRunReal0 = 0, kRunReal1, kRunImag0, kRunImag1, kRunCplx, kRunMulti, kRuns };
+constexpr int kRowsMax = 8;   // amplitudes per thread of the tiled kernels, at most
 
-struct GroupRuns { uint32_t xh, b[kRuns + 1]; };
+struct GroupRuns { uint32_t xh, b[kRuns * kRowsMax + 1]; };
 
 static inline uint32_t pext_even(uint64_t b)   // bit 2s -> bit s
 {
@@ -108,6 +114,9 @@ int build_tables(int len, int tb, int threads, const uint64_t *bits, const doubl
     if (tb) {
         const int lb = __builtin_ctz((unsigned)threads);
         const uint32_t rpmask = (1u << (tb - lb)) - 1;
+        if (rpmask + 1 > (uint32_t)kRowsMax)
+            return set_error(RP_ERR_INVALID_ARGUMENT, "tiling with more than %d rows per thread",
+                             kRowsMax);
         size_t i = 0;
         while (i < v.size()) {
             const uint32_t xh = v[i].x >> tb;
@@ -124,6 +133,8 @@ int build_tables(int len, int tb, int threads, const uint64_t *bits, const doubl
                     S.z = v[i].z;
                     S.cr = v[i].cr;
                     S.ci = v[i].ci;
+                    S.fr = (float)v[i].cr;
+                    S.fi = (float)v[i].ci;
                     const bool rowz = ((S.z >> lb) & rpmask) != 0;
                     if (v[i].ci == 0) { S.kind = kSubReal; cls = rowz ? kRunReal1 : kRunReal0; }
                     else if (v[i].cr == 0) { S.kind = kSubImag; cls = rowz ? kRunImag1 : kRunImag0; }
@@ -141,10 +152,18 @@ int build_tables(int len, int tb, int threads, const uint64_t *bits, const doubl
             GroupRuns G;
             G.xh = xh;
             for (int k = 0; k < kRuns; k++) {
-                G.b[k] = (uint32_t)subs.size();
+                std::stable_sort(run[k].begin(), run[k].end(),
+                                 [&](const SubRec<A> &a, const SubRec<A> &b) {
+                                     return (a.xl >> lb) < (b.xl >> lb);
+                                 });
+                size_t q = 0;
+                for (int xr = 0; xr < kRowsMax; xr++) {
+                    G.b[k * kRowsMax + xr] = (uint32_t)(subs.size() + q);
+                    while (q < run[k].size() && (run[k][q].xl >> lb) == (uint32_t)xr) q++;
+                }
                 subs.insert(subs.end(), run[k].begin(), run[k].end());
             }
-            G.b[kRuns] = (uint32_t)subs.size();
+            G.b[kRuns * kRowsMax] = (uint32_t)subs.size();
             groups.push_back(G);
         }
     }
