@@ -38,12 +38,12 @@ __device__ inline uint32_t parity32(uint32_t v) { return __popc(v) & 1u; }
 // Sub classes, in the order a group stores its subs: single-term real c with
 // no phase bit in the row bits, real with row bits, imaginary (same split),
 // complex, and several terms per flip mask.  Within a class the subs are
-// sorted by their row permutation xr = (x mod 2^b) / threads (< 8 for every
+// sorted by their row permutation xr = (x mod 2^b) / threads (< 16 for every
 // tiling), so that a kernel can walk class k, permutation q as the range
 // [b[k*kRowsMax + q], b[k*kRowsMax + q + 1]) with q a compile-time constant;
 // class k as a whole is [b[k*kRowsMax], b[(k+1)*kRowsMax]).
 enum : uint32_t { kRunReal0 = 0, kRunReal1, kRunImag0, kRunImag1, kRunCplx, kRunMulti, kRuns };
-constexpr int kRowsMax = 8;   // amplitudes per thread of the tiled kernels, at most
+constexpr int kRowsMax = 16;  // amplitudes per thread of the tiled kernels, at most
 
 struct GroupRuns { uint32_t xh, b[kRuns * kRowsMax + 1]; };
 
