@@ -126,7 +126,8 @@ __device__ inline void tile_dma(const T *__restrict__ in, uint64_t src, T *dst)
     }
 }
 
-// a sub's coefficient in the partial-sum type P
+// a sub's coefficient in the partial-sum type P (float copies for FP32
+// partial sums of FP64-coefficient layouts: no conversion in the loop)
 template <typename P, typename A>
 __device__ __forceinline__ P sub_re(const SubRec<A> &S)
 {
@@ -141,10 +142,7 @@ __device__ __forceinline__ P sub_im(const SubRec<A> &S)
 }
 
 // Apply every sub of group G from the staged input tile `buf` (LDS) to the
-// RP rows of this thread, summing in P (A, or float for group accumulation).
-// The subs of each class come sorted by their row permutation xr; walking
-// them per xr makes r ^ xr a compile-time constant, so each LDS read is the
-// lane's base address plus an immediate offset.
+// RP accumulated rows of this thread.
 template <typename R, typename A, int TB, int NT, typename P = A>
 __device__ __forceinline__ void apply_group(const GroupRuns &G, const typename CxA<R>::T *buf,
                                             const SubRec<A> *__restrict__ subs,
@@ -155,114 +153,104 @@ __device__ __forceinline__ void apply_group(const GroupRuns &G, const typename C
     using T = typename CxA<R>::T;
     constexpr int RP = (1 << TB) / NT;
     constexpr int LB = NT == 1024 ? 10 : NT == 512 ? 9 : NT == 256 ? 8 : 7;
-    static_assert(RP <= kRowsMax, "rows per thread");
 
-#define RP_SUBS(cls) \
-    for (uint32_t s = G.b[(cls) * kRowsMax + XR]; s < G.b[(cls) * kRowsMax + XR + 1]; s++)
 #define RP_SUB_PROLOGUE                                                            \
     const SubRec<A> S = subs[s];                                               \
     const T *row = buf + (tid ^ (S.xl & (NT - 1)));                            \
+    const uint32_t xr = S.xl >> LB;                                            \
     const bool lneg = parity32((t0 | tid) & S.z);                              \
     const uint32_t zr = (S.z >> LB) & (RP - 1);                                \
     (void)zr;
+    // real c, no phase bit among the row bits
+    for (uint32_t s = G.b[kRunReal0]; s < G.b[kRunReal0 + 1]; s++) {
+        RP_SUB_PROLOGUE
+        const P c0 = sub_re<P>(S), c = lneg ? -c0 : c0;
 #pragma unroll
-    for (int XR = 0; XR < RP; XR++) {
-        // real c, no phase bit among the row bits
-        RP_SUBS(kRunReal0) {
-            RP_SUB_PROLOGUE
-            const P c0 = sub_re<P>(S);
-            const P c = lneg ? -c0 : c0;
-#pragma unroll
-            for (int r = 0; r < RP; r++) {
-                const T v = row[(r ^ XR) * NT];
-                accr[r] += c * (P)v.x;
-                acci[r] += c * (P)v.y;
-            }
+        for (int r = 0; r < RP; r++) {
+            const T v = row[(r ^ xr) * NT];
+            accr[r] += c * (P)v.x;
+            acci[r] += c * (P)v.y;
         }
-        RP_SUBS(kRunReal1) {
-            RP_SUB_PROLOGUE
-            const P c0 = sub_re<P>(S);
-            const P c = lneg ? -c0 : c0;
+    }
+    for (uint32_t s = G.b[kRunReal1]; s < G.b[kRunReal1 + 1]; s++) {
+        RP_SUB_PROLOGUE
+        const P c0 = sub_re<P>(S), c = lneg ? -c0 : c0;
 #pragma unroll
-            for (int r = 0; r < RP; r++) {
-                const T v = row[(r ^ XR) * NT];
-                const P cc = parity32(r & zr) ? -c : c;
-                accr[r] += cc * (P)v.x;
-                acci[r] += cc * (P)v.y;
-            }
+        for (int r = 0; r < RP; r++) {
+            const T v = row[(r ^ xr) * NT];
+            const P cc = parity32(r & zr) ? -c : c;
+            accr[r] += cc * (P)v.x;
+            acci[r] += cc * (P)v.y;
         }
-        // imaginary c = i ci: (i ci)(vr + i vi) = -ci vi + i ci vr
-        RP_SUBS(kRunImag0) {
-            RP_SUB_PROLOGUE
-            const P c0 = sub_im<P>(S);
-            const P c = lneg ? -c0 : c0;
+    }
+    // imaginary c = i ci: (i ci)(vr + i vi) = -ci vi + i ci vr
+    for (uint32_t s = G.b[kRunImag0]; s < G.b[kRunImag0 + 1]; s++) {
+        RP_SUB_PROLOGUE
+        const P c0 = sub_im<P>(S), c = lneg ? -c0 : c0;
 #pragma unroll
-            for (int r = 0; r < RP; r++) {
-                const T v = row[(r ^ XR) * NT];
-                accr[r] -= c * (P)v.y;
-                acci[r] += c * (P)v.x;
-            }
+        for (int r = 0; r < RP; r++) {
+            const T v = row[(r ^ xr) * NT];
+            accr[r] -= c * (P)v.y;
+            acci[r] += c * (P)v.x;
         }
-        RP_SUBS(kRunImag1) {
-            RP_SUB_PROLOGUE
-            const P c0 = sub_im<P>(S);
-            const P c = lneg ? -c0 : c0;
+    }
+    for (uint32_t s = G.b[kRunImag1]; s < G.b[kRunImag1 + 1]; s++) {
+        RP_SUB_PROLOGUE
+        const P c0 = sub_im<P>(S), c = lneg ? -c0 : c0;
 #pragma unroll
-            for (int r = 0; r < RP; r++) {
-                const T v = row[(r ^ XR) * NT];
-                const P cc = parity32(r & zr) ? -c : c;
-                accr[r] -= cc * (P)v.y;
-                acci[r] += cc * (P)v.x;
-            }
+        for (int r = 0; r < RP; r++) {
+            const T v = row[(r ^ xr) * NT];
+            const P cc = parity32(r & zr) ? -c : c;
+            accr[r] -= cc * (P)v.y;
+            acci[r] += cc * (P)v.x;
         }
-        RP_SUBS(kRunCplx) {
-            RP_SUB_PROLOGUE
-            const P r0 = sub_re<P>(S), i0 = sub_im<P>(S);
-            const P cr = lneg ? -r0 : r0;
-            const P ci = lneg ? -i0 : i0;
+    }
+    for (uint32_t s = G.b[kRunCplx]; s < G.b[kRunCplx + 1]; s++) {
+        RP_SUB_PROLOGUE
+        const P r0 = sub_re<P>(S), i0 = sub_im<P>(S);
+        const P cr = lneg ? -r0 : r0, ci = lneg ? -i0 : i0;
 #pragma unroll
-            for (int r = 0; r < RP; r++) {
-                const T v = row[(r ^ XR) * NT];
-                const bool n = parity32(r & zr);
-                const P a = n ? -cr : cr, c = n ? -ci : ci;
-                accr[r] += a * (P)v.x - c * (P)v.y;
-                acci[r] += a * (P)v.y + c * (P)v.x;
-            }
+        for (int r = 0; r < RP; r++) {
+            const T v = row[(r ^ xr) * NT];
+            const bool n = parity32(r & zr);
+            const P a = n ? -cr : cr, c = n ? -ci : ci;
+            accr[r] += a * (P)v.x - c * (P)v.y;
+            acci[r] += a * (P)v.y + c * (P)v.x;
         }
-        // several terms share x: d(m) = sum_t c_t (-1)^popcount(m & z_t),
-        // formed for four rows at a time to bound register use
-        RP_SUBS(kRunMulti) {
-            const SubRec<A> S = subs[s];
-            const T *row = buf + (tid ^ (S.xl & (NT - 1)));
-            constexpr int RC = RP < 4 ? RP : 4;
+    }
+    // several terms share x: d(m) = sum_t c_t (-1)^popcount(m & z_t),
+    // formed for four rows at a time to bound register use
+    for (uint32_t s = G.b[kRunMulti]; s < G.b[kRunMulti + 1]; s++) {
+        const SubRec<A> S = subs[s];
+        const T *row = buf + (tid ^ (S.xl & (NT - 1)));
+        const uint32_t xr = S.xl >> LB;
+        constexpr int RC = RP < 4 ? RP : 4;
 #pragma unroll
-            for (int r0 = 0; r0 < RP; r0 += RC) {
-                P dr[RC], di[RC];
+        for (int r0 = 0; r0 < RP; r0 += RC) {
+            P dr[RC], di[RC];
 #pragma unroll
-                for (int q = 0; q < RC; q++) { dr[q] = 0; di[q] = 0; }
-                for (uint32_t t = S.term_begin; t < S.term_end; t++) {
-                    const TermRec<A> C = terms[t];
-                    const bool lneg = parity32((t0 | tid) & C.z);
-                    const P cr = (P)(lneg ? -C.cr : C.cr);
-                    const P ci = (P)(lneg ? -C.ci : C.ci);
-                    const uint32_t zr = (C.z >> LB) & (RP - 1);
-#pragma unroll
-                    for (int q = 0; q < RC; q++) {
-                        const bool n = parity32((r0 + q) & zr);
-                        dr[q] += n ? -cr : cr;
-                        di[q] += n ? -ci : ci;
-                    }
-                }
+            for (int q = 0; q < RC; q++) { dr[q] = 0; di[q] = 0; }
+            for (uint32_t t = S.term_begin; t < S.term_end; t++) {
+                const TermRec<A> C = terms[t];
+                const bool lneg = parity32((t0 | tid) & C.z);
+                const P cr = (P)(lneg ? -C.cr : C.cr);
+                const P ci = (P)(lneg ? -C.ci : C.ci);
+                const uint32_t zr = (C.z >> LB) & (RP - 1);
 #pragma unroll
                 for (int q = 0; q < RC; q++) {
-                    const T v = row[((r0 + q) ^ XR) * NT];
-                    accr[r0 + q] += dr[q] * (P)v.x - di[q] * (P)v.y;
-                    acci[r0 + q] += dr[q] * (P)v.y + di[q] * (P)v.x;
+                    const bool n = parity32((r0 + q) & zr);
+                    dr[q] += n ? -cr : cr;
+                    di[q] += n ? -ci : ci;
                 }
+            }
+#pragma unroll
+            for (int q = 0; q < RC; q++) {
+                const T v = row[((r0 + q) ^ xr) * NT];
+                accr[r0 + q] += dr[q] * (P)v.x - di[q] * (P)v.y;
+                acci[r0 + q] += dr[q] * (P)v.y + di[q] * (P)v.x;
             }
         }
     }
-#undef RP_SUBS
 #undef RP_SUB_PROLOGUE
 }
 

@@ -114,7 +114,7 @@ __device__ __forceinline__ void batch_group(const GroupRuns &G, const Elt<R, NS>
     constexpr int RP = (1 << TB) / NT;
     constexpr int LB = NT == 1024 ? 10 : NT == 512 ? 9 : NT == 256 ? 8 : 7;
     // real c (both row-parity classes): c * v
-    for (uint32_t s = G.b[kRunReal0 * kRowsMax]; s < G.b[(kRunReal1 + 1) * kRowsMax]; s++) {
+    for (uint32_t s = G.b[kRunReal0]; s < G.b[kRunReal1 + 1]; s++) {
         const SubRec<A> S = subs[s];
         const E *row = buf + (tid ^ (S.xl & (NT - 1)));
         const uint32_t xr = S.xl >> LB, zr = (S.z >> LB) & (RP - 1);
@@ -131,7 +131,7 @@ __device__ __forceinline__ void batch_group(const GroupRuns &G, const Elt<R, NS>
         }
     }
     // imaginary c = i ci: -ci vi + i ci vr
-    for (uint32_t s = G.b[kRunImag0 * kRowsMax]; s < G.b[(kRunImag1 + 1) * kRowsMax]; s++) {
+    for (uint32_t s = G.b[kRunImag0]; s < G.b[kRunImag1 + 1]; s++) {
         const SubRec<A> S = subs[s];
         const E *row = buf + (tid ^ (S.xl & (NT - 1)));
         const uint32_t xr = S.xl >> LB, zr = (S.z >> LB) & (RP - 1);
@@ -147,7 +147,7 @@ __device__ __forceinline__ void batch_group(const GroupRuns &G, const Elt<R, NS>
             }
         }
     }
-    for (uint32_t s = G.b[kRunCplx * kRowsMax]; s < G.b[(kRunCplx + 1) * kRowsMax]; s++) {
+    for (uint32_t s = G.b[kRunCplx]; s < G.b[kRunCplx + 1]; s++) {
         const SubRec<A> S = subs[s];
         const E *row = buf + (tid ^ (S.xl & (NT - 1)));
         const uint32_t xr = S.xl >> LB, zr = (S.z >> LB) & (RP - 1);
@@ -166,7 +166,7 @@ __device__ __forceinline__ void batch_group(const GroupRuns &G, const Elt<R, NS>
         }
     }
     // several terms share x: d(m) = sum_t c_t (-1)^popcount(m & z_t)
-    for (uint32_t s = G.b[kRunMulti * kRowsMax]; s < G.b[(kRunMulti + 1) * kRowsMax]; s++) {
+    for (uint32_t s = G.b[kRunMulti]; s < G.b[kRunMulti + 1]; s++) {
         const SubRec<A> S = subs[s];
         const E *row = buf + (tid ^ (S.xl & (NT - 1)));
         const uint32_t xr = S.xl >> LB;
@@ -607,7 +607,7 @@ int build_batch_tables(rp_batch *b, int tb, int nt, const uint64_t *bits, const 
             P.g0 = (uint32_t)groups.size();
             const uint32_t s0 = (uint32_t)subs.size(), t0 = (uint32_t)terms.size();
             for (auto G : t.groups) {
-                for (int k = 0; k <= kRuns * kRowsMax; k++) G.b[k] += s0;
+                for (int k = 0; k <= kRuns; k++) G.b[k] += s0;
                 groups.push_back(G);
             }
             for (auto S : t.subs) {

@@ -37,15 +37,10 @@ __device__ inline uint32_t parity32(uint32_t v) { return __popc(v) & 1u; }
 
 // Sub classes, in the order a group stores its subs: single-term real c with
 // no phase bit in the row bits, real with row bits, imaginary (same split),
-// complex, and several terms per flip mask.  Within a class the subs are
-// sorted by their row permutation xr = (x mod 2^b) / threads (< 16 for every
-// tiling), so that a kernel can walk class k, permutation q as the range
-// [b[k*kRowsMax + q], b[k*kRowsMax + q + 1]) with q a compile-time constant;
-// class k as a whole is [b[k*kRowsMax], b[(k+1)*kRowsMax]).
+// complex, and several terms per flip mask.
 enum : uint32_t { kRunReal0 = 0, kRunReal1, kRunImag0, kRunImag1, kRunCplx, kRunMulti, kRuns };
-constexpr int kRowsMax = 16;  // amplitudes per thread of the tiled kernels, at most
 
-struct GroupRuns { uint32_t xh, b[kRuns * kRowsMax + 1]; };
+struct GroupRuns { uint32_t xh, b[kRuns + 1]; };
 
 static inline uint32_t pext_even(uint64_t b)   // bit 2s -> bit s
 {
@@ -114,9 +109,6 @@ int build_tables(int len, int tb, int threads, const uint64_t *bits, const doubl
     if (tb) {
         const int lb = __builtin_ctz((unsigned)threads);
         const uint32_t rpmask = (1u << (tb - lb)) - 1;
-        if (rpmask + 1 > (uint32_t)kRowsMax)
-            return set_error(RP_ERR_INVALID_ARGUMENT, "tiling with more than %d rows per thread",
-                             kRowsMax);
         size_t i = 0;
         while (i < v.size()) {
             const uint32_t xh = v[i].x >> tb;
@@ -152,18 +144,10 @@ int build_tables(int len, int tb, int threads, const uint64_t *bits, const doubl
             GroupRuns G;
             G.xh = xh;
             for (int k = 0; k < kRuns; k++) {
-                std::stable_sort(run[k].begin(), run[k].end(),
-                                 [&](const SubRec<A> &a, const SubRec<A> &b) {
-                                     return (a.xl >> lb) < (b.xl >> lb);
-                                 });
-                size_t q = 0;
-                for (int xr = 0; xr < kRowsMax; xr++) {
-                    G.b[k * kRowsMax + xr] = (uint32_t)(subs.size() + q);
-                    while (q < run[k].size() && (run[k][q].xl >> lb) == (uint32_t)xr) q++;
-                }
+                G.b[k] = (uint32_t)subs.size();
                 subs.insert(subs.end(), run[k].begin(), run[k].end());
             }
-            G.b[kRuns * kRowsMax] = (uint32_t)subs.size();
+            G.b[kRuns] = (uint32_t)subs.size();
             groups.push_back(G);
         }
     }
