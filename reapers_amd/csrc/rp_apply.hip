@@ -287,7 +287,8 @@ __device__ __forceinline__ void pipe_phase(
     const GroupRuns *__restrict__ groups, const SubRec<A> *__restrict__ subs,
     const TermRec<A> *__restrict__ terms, uint32_t t0, uint32_t tid,
     typename CxA<R>::T *b0, typename CxA<R>::T *b1, typename CxA<R>::T *b2,
-    typename CxA<R>::T *b3, A (&accr)[(1 << TB) / NT], A (&acci)[(1 << TB) / NT])
+    typename CxA<R>::T *b3, A (&accr)[(1 << TB) / NT], A (&acci)[(1 << TB) / NT],
+    float (&gr)[(1 << TB) / NT], float (&gi)[(1 << TB) / NT])
 {
     using T = typename CxA<R>::T;
     constexpr int PT = ((1 << TB) * (int)sizeof(T)) / (16 * NT);   // DMA ops per thread per tile
@@ -306,16 +307,25 @@ __device__ __forceinline__ void pipe_phase(
     if (g + D - 1 < ngroups)
         tile_dma<T, TB, NT>(in, t0 ^ (groups[g + D - 1].xh << TB), bufs[(K + D - 1) % D]);
     if constexpr (GA) {
-        // the group's terms summed in FP32, the group sums added in FP64
-        constexpr int RP = (1 << TB) / NT;
-        float gr[RP], gi[RP];
-#pragma unroll
-        for (int r = 0; r < RP; r++) { gr[r] = 0; gi[r] = 0; }
+        // the terms of a pair of groups summed in FP32 (gr, gi), the pair
+        // sums added to the FP64 accumulators by the caller
         apply_group<R, A, TB, NT, float>(groups[g], bufs[K], subs, terms, t0, tid, gr, gi);
-#pragma unroll
-        for (int r = 0; r < RP; r++) { accr[r] += (A)gr[r]; acci[r] += (A)gi[r]; }
     } else {
         apply_group<R, A, TB, NT>(groups[g], bufs[K], subs, terms, t0, tid, accr, acci);
+    }
+}
+
+// add the FP32 partial sums to the FP64 accumulators and clear them
+template <typename A, int RP>
+__device__ __forceinline__ void flush_partials(A (&accr)[RP], A (&acci)[RP], float (&gr)[RP],
+                                               float (&gi)[RP])
+{
+#pragma unroll
+    for (int r = 0; r < RP; r++) {
+        accr[r] += (A)gr[r];
+        acci[r] += (A)gi[r];
+        gr[r] = 0;
+        gi[r] = 0;
     }
 }
 
@@ -345,8 +355,9 @@ k_apply_tiled(const typename CxA<R>::T *__restrict__ in, typename CxA<R>::T *__r
     const uint32_t tid = threadIdx.x;
 
     A accr[RP], acci[RP];
+    float gr[RP], gi[RP];      // FP32 partial sums (group accumulation)
 #pragma unroll
-    for (int r = 0; r < RP; r++) { accr[r] = 0; acci[r] = 0; }
+    for (int r = 0; r < RP; r++) { accr[r] = 0; acci[r] = 0; gr[r] = 0; gi[r] = 0; }
 
     T *const ring[4] = {tile0, tile1, tile2, tile3};
 #pragma unroll
@@ -354,21 +365,24 @@ k_apply_tiled(const typename CxA<R>::T *__restrict__ in, typename CxA<R>::T *__r
         if (q < ngroups) tile_dma<T, TB, NT>(in, t0 ^ (groups[q].xh << TB), ring[q]);
     for (int g = 0; g < ngroups; g += D) {
         pipe_phase<R, A, TB, NT, D, 0, GA>(g, ngroups, in, groups, subs, terms, t0, tid, tile0, tile1,
-                                       tile2, tile3, accr, acci);
+                                       tile2, tile3, accr, acci, gr, gi);
         if (g + 1 >= ngroups) break;
         pipe_phase<R, A, TB, NT, D, 1, GA>(g + 1, ngroups, in, groups, subs, terms, t0, tid, tile0,
-                                       tile1, tile2, tile3, accr, acci);
+                                       tile1, tile2, tile3, accr, acci, gr, gi);
         if constexpr (D > 2) {
             if (g + 2 >= ngroups) break;
             pipe_phase<R, A, TB, NT, D, 2, GA>(g + 2, ngroups, in, groups, subs, terms, t0, tid, tile0,
-                                           tile1, tile2, tile3, accr, acci);
+                                           tile1, tile2, tile3, accr, acci, gr, gi);
         }
         if constexpr (D > 3) {
             if (g + 3 >= ngroups) break;
             pipe_phase<R, A, TB, NT, D, 3, GA>(g + 3, ngroups, in, groups, subs, terms, t0, tid, tile0,
-                                           tile1, tile2, tile3, accr, acci);
+                                           tile1, tile2, tile3, accr, acci, gr, gi);
         }
+        // FP32 partial sums of this round's D groups into the FP64 accumulators
+        if constexpr (GA) flush_partials(accr, acci, gr, gi);
     }
+    if constexpr (GA) flush_partials(accr, acci, gr, gi);
 #pragma unroll
     for (int r = 0; r < RP; r++) {
         const uint64_t m = (uint64_t)t0 + r * NT + tid;

@@ -174,5 +174,6 @@ def test_group_accumulation_matches_oracle(gpu, orc, tiling):
     got = vout.to_array(np.complex64, len(psi)).astype(np.complex128)
     err = np.abs(got - ref).max() / np.abs(ref).max()
     print("group accumulation %s: %.2e" % (tiling, err), flush=True)
-    assert err < 2e-7
+    # a few FP32 ulps of the largest amplitude (FP32 accumulation: ~1e-6)
+    assert err < 4e-7
     gpu.check(gpu.hip().rp_terms_free(h))
