@@ -44,6 +44,14 @@ def _bind(L):
         L.or_syk_gen_f64.restype = _c.c_int64
         L.or_syk_gen_f64.argtypes = [_c.c_int, _c.c_double, _c.c_double, _c.c_int, _c.c_uint32,
                                      _c.c_int, _vp, _vp, _c.c_int64]
+        L.or_majorana_block_f32.argtypes = [_c.c_int, _c.c_int, _c.c_int, _c.c_int, _vp, _vp]
+        L.or_majorana_block_f64.argtypes = [_c.c_int, _c.c_int, _c.c_int, _c.c_int, _vp, _vp]
+        L.or_syk_block_gen_f32.restype = _c.c_int64
+        L.or_syk_block_gen_f32.argtypes = [_c.c_int, _c.c_float, _c.c_float, _c.c_int,
+                                           _c.c_uint32, _c.c_int, _c.c_int, _vp, _vp, _c.c_int64]
+        L.or_syk_block_gen_f64.restype = _c.c_int64
+        L.or_syk_block_gen_f64.argtypes = [_c.c_int, _c.c_double, _c.c_double, _c.c_int,
+                                           _c.c_uint32, _c.c_int, _c.c_int, _vp, _vp, _c.c_int64]
         L.or_majorana_anticomm_f64.restype = _c.c_int64
         L.or_majorana_anticomm_f64.argtypes = [_c.c_int, _c.c_int, _c.c_int, _c.c_int, _vp, _vp,
                                                _c.c_int64]
@@ -208,6 +216,36 @@ def syk_terms(N, sp=0.0, J=1.0, seed=0, fp=64, std_gamma=True, regularize=False)
             c = np.zeros(2 * cap, np.float64)
             n = lib().or_syk_gen_f64(N, sp, J, int(std_gamma), seed, int(regularize), _p(bits),
                                      _p(c), cap)
+        if n >= 0:
+            return bits[:n].copy(), _join(c[:2 * n].astype(np.float64))
+        cap = -n
+
+
+def majorana_block(N, n, which, fp=64, std_gamma=True):
+    """Block which (0 = LR, 1 = RL) of the parity-block Majorana n: (bits, coeff)."""
+    bits = np.zeros(1, np.uint64)
+    if fp == 32:
+        c = np.zeros(2, np.float32)
+        lib().or_majorana_block_f32(N, n, int(std_gamma), int(which), _p(bits), _p(c))
+    else:
+        c = np.zeros(2, np.float64)
+        lib().or_majorana_block_f64(N, n, int(std_gamma), int(which), _p(bits), _p(c))
+    return int(bits[0]), complex(c[0], c[1])
+
+
+def syk_block_terms(N, which, sp=0.0, J=1.0, seed=0, fp=64, std_gamma=True, regularize=False):
+    """Block which (0 = LL, 1 = RR) of the parity-block SYK Hamiltonian."""
+    cap = 1 << 12
+    while True:
+        bits = np.zeros(cap, np.uint64)
+        if fp == 32:
+            c = np.zeros(2 * cap, np.float32)
+            n = lib().or_syk_block_gen_f32(N, sp, J, int(std_gamma), seed, int(regularize),
+                                           int(which), _p(bits), _p(c), cap)
+        else:
+            c = np.zeros(2 * cap, np.float64)
+            n = lib().or_syk_block_gen_f64(N, sp, J, int(std_gamma), seed, int(regularize),
+                                           int(which), _p(bits), _p(c), cap)
         if n >= 0:
             return bits[:n].copy(), _join(c[:2 * n].astype(np.float64))
         cap = -n

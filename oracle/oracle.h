@@ -74,6 +74,12 @@ void     or_pauli_sparse_matrix(uint64_t bits, int len, uint32_t *cols, uint8_t 
 /* Majorana chi_n on N/2 sites: one term */
 void or_majorana_f32(int N, int n, int std_gamma, uint64_t *bits, float *coeff2);
 void or_majorana_f64(int N, int n, int std_gamma, uint64_t *bits, double *coeff2);
+void or_majorana_block_f32(int N, int n, int std_gamma, int which, uint64_t *bits, float *coeff2);
+void or_majorana_block_f64(int N, int n, int std_gamma, int which, uint64_t *bits, double *coeff2);
+int64_t or_syk_block_gen_f32(int N, float sp, float J, int std_gamma, uint32_t seed,
+                             int regularize, int which, uint64_t *bits, float *coeff, int64_t cap);
+int64_t or_syk_block_gen_f64(int N, double sp, double J, int std_gamma, uint32_t seed,
+                             int regularize, int which, uint64_t *bits, double *coeff, int64_t cap);
 /* SYK::gen_ham(std::mt19937(seed), regularize); returns the number of terms,
  * or -needed if cap is too small; coeff is interleaved (re, im). */
 int64_t or_syk_gen_f32(int N, float sp, float J, int std_gamma, uint32_t seed,

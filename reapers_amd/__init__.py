@@ -169,6 +169,31 @@ def host():
             "rph_state_inner": ([_vp, _vp, _c.POINTER(_dbl), _c.POINTER(_dbl)], _c.c_int),
             "rph_state_norm": ([_vp, _c.POINTER(_dbl)], _c.c_int),
             "rph_state_ground": ([_vp, _vp, _c.c_int, _dbl, _c.c_int, _c.POINTER(_dbl)], _c.c_int),
+            "rph_syk_block_gen": ([_c.c_int, _c.c_int, _dbl, _dbl, _c.c_int, _c.c_uint32,
+                                   _c.c_int, _c.c_int, _vp, _vp, _i64, _c.POINTER(_i64)],
+                                  _c.c_int),
+            "rph_majorana_block": ([_c.c_int, _c.c_int, _c.c_int, _c.c_int, _c.c_int, _vp, _vp,
+                                    _i64, _c.POINTER(_i64)], _c.c_int),
+            "rph_block_anticomm": ([_c.c_int, _c.c_int, _c.c_int, _c.c_int, _c.c_int, _c.c_int,
+                                    _vp, _vp, _i64, _c.POINTER(_i64), _c.POINTER(_c.c_int)],
+                                   _c.c_int),
+            "rph_block_state_new": ([_c.c_int, _c.c_int], _vp),
+            "rph_block_state_free": ([_vp], None),
+            "rph_block_state_load": ([_vp, _vp, _vp], _c.c_int),
+            "rph_block_state_store": ([_vp, _vp, _vp, _c.POINTER(_c.c_int),
+                                       _c.POINTER(_c.c_int)], _c.c_int),
+            "rph_block_state_random": ([_vp, _u64], _c.c_int),
+            "rph_block_state_get": ([_vp, _u64, _c.POINTER(_dbl), _c.POINTER(_dbl)], _c.c_int),
+            "rph_block_state_norm": ([_vp, _c.POINTER(_dbl)], _c.c_int),
+            "rph_block_state_apply": ([_vp, _vp, _vp, _vp, _vp], _c.c_int),
+            "rph_block_state_evolve": ([_vp, _vp, _vp, _dbl, _dbl, _c.c_int, _dbl,
+                                        _c.POINTER(EvolveStats)], _c.c_int),
+            "rph_block_state_ground": ([_vp, _vp, _vp, _c.c_int, _dbl, _c.c_int,
+                                        _c.POINTER(_dbl), _c.POINTER(_c.c_int)], _c.c_int),
+            "rph_syk_block_npoint": ([_c.c_int, _c.c_int, _c.c_int, _dbl, _dbl, _c.c_uint32,
+                                      _c.c_int, _c.c_int, _vp, _u64, _dbl, _dbl, _c.c_int,
+                                      _c.c_int, _dbl, _vp, _c.POINTER(_dbl),
+                                      _c.POINTER(EvolveStats)], _c.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(lib, name)
@@ -226,6 +251,42 @@ def majorana(N, n, fp=32, std_gamma=True):
     co = np.zeros(2, np.float64)
     hcheck(host().rph_majorana(fp, N, n, int(std_gamma), _ptr(bits), _ptr(co)))
     return int(bits[0]), complex(co[0], co[1])
+
+
+def _terms_out(fn, cap=1 << 12):
+    """Call fn(bits, coeff, cap, &n) until the term list fits; (bits, complex)."""
+    while True:
+        bits = np.zeros(cap, np.uint64)
+        co = np.zeros(2 * cap, np.float64)
+        n = _i64()
+        fn(_ptr(bits), _ptr(co), cap, _c.byref(n))
+        if n.value >= 0:
+            return bits[:n.value].copy(), (co[0:2 * n.value:2] + 1j * co[1:2 * n.value:2])
+        cap = -n.value
+
+
+def syk_block_terms(N, which, sp=0.0, J=1.0, seed=0, fp=32, std_gamma=True, regularize=False):
+    """Block which (0 = LL, 1 = RR) of SYKBlockForm(N, sp, J).gen_ham(
+    std::mt19937(seed)): a term list on N/2-1 sites."""
+    return _terms_out(lambda b, c, cap, n: hcheck(host().rph_syk_block_gen(
+        fp, N, sp, J, int(std_gamma), seed, int(regularize), int(which), b, c, cap, n)))
+
+
+def majorana_block(N, n, which, fp=32, std_gamma=True):
+    """Block which (0 = LR, 1 = RL) of FermionOpBlockForm(N, n): (bits, coeff)."""
+    b, c = _terms_out(lambda bb, cc, cap, k: hcheck(host().rph_majorana_block(
+        fp, N, n, int(std_gamma), int(which), bb, cc, cap, k)), cap=4)
+    assert len(b) == 1
+    return int(b[0]), complex(c[0])
+
+
+def block_anticomm(N, i, j, which, fp=64, std_gamma=True):
+    """Block which (0 = LL, 1 = RR) of chi_i chi_j + chi_j chi_i in block
+    form: (bits, coeff, null)."""
+    null = _c.c_int()
+    b, c = _terms_out(lambda bb, cc, cap, k: hcheck(host().rph_block_anticomm(
+        fp, N, i, j, int(std_gamma), int(which), bb, cc, cap, k, _c.byref(null))), cap=8)
+    return b, c, bool(null.value)
 
 
 def random_stream(kind, n, seed, fp=64, a=0.0, b=0.0):
@@ -390,6 +451,131 @@ class DeviceState:
             self.close()
         except Exception:
             pass
+
+
+class BlockState:
+    """A REAPERS BlockState<Fp, GPUImpl>: the L and R parity sectors of
+    2^length amplitudes each (full vector: L first)."""
+
+    def __init__(self, length, fp=32):
+        self.len = int(length)
+        self.fp = fp
+        self.half = 1 << self.len
+        self._s = host().rph_block_state_new(fp, self.len)
+        if not self._s:
+            raise RpError(-1, host().rph_last_error().decode())
+
+    @property
+    def dtype(self):
+        return np.complex64 if self.fp == 32 else np.complex128
+
+    def load(self, L, R):
+        """Load the sectors; None makes a sector null."""
+        arrs = []
+        for v in (L, R):
+            if v is None:
+                arrs.append(None)
+            else:
+                a = np.ascontiguousarray(v, dtype=self.dtype)
+                assert a.shape == (self.half,)
+                arrs.append(a)
+        hcheck(host().rph_block_state_load(self._s, *[None if a is None else _ptr(a)
+                                                      for a in arrs]))
+
+    def store(self):
+        """(L, R), None for a null sector."""
+        L = np.empty(self.half, self.dtype)
+        R = np.empty(self.half, self.dtype)
+        nl, nr = _c.c_int(), _c.c_int()
+        hcheck(host().rph_block_state_store(self._s, _ptr(L), _ptr(R), _c.byref(nl),
+                                            _c.byref(nr)))
+        return (None if nl.value else L), (None if nr.value else R)
+
+    def random(self, seed):
+        hcheck(host().rph_block_state_random(self._s, seed))
+
+    def __getitem__(self, i):
+        re, im = _dbl(), _dbl()
+        hcheck(host().rph_block_state_get(self._s, int(i), _c.byref(re), _c.byref(im)))
+        return complex(re.value, im.value)
+
+    def norm(self):
+        n = _dbl()
+        hcheck(host().rph_block_state_norm(self._s, _c.byref(n)))
+        return n.value
+
+    def apply(self, LL=None, LR=None, RL=None, RR=None):
+        """state *= the block operator with these blocks (Hamiltonian
+        handles; None = null block)."""
+        h = [None if x is None else x._h for x in (LL, LR, RL, RR)]
+        hcheck(host().rph_block_state_apply(self._s, *h))
+
+    def evolve(self, LL, RR, t, beta=0.0, krydim=5, tol=0.0):
+        st = EvolveStats()
+        hcheck(host().rph_block_state_evolve(self._s, LL._h, RR._h, t, beta, krydim, tol,
+                                             _c.byref(st)))
+        return st
+
+    def ground_state(self, LL, RR, krydim=5, eps=1e-10, maxiter=0):
+        """(energy, sector kept: 0 = L, 1 = R)."""
+        e, k = _dbl(), _c.c_int()
+        hcheck(host().rph_block_state_ground(self._s, LL._h, RR._h, krydim, eps, maxiter,
+                                             _c.byref(e), _c.byref(k)))
+        return e.value, k.value
+
+    def close(self):
+        if self._s:
+            host().rph_block_state_free(self._s)
+            self._s = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def syk_block_npoint(N, kind="otoc", sp=0.0, J=1.0, seed=0, regularize=False, otoc_idx=-1,
+                     init=None, state_seed=1, beta=0.0, tmax=1.0, nsteps=10, krylov_dim=5,
+                     tol=0.0, fp=32):
+    """One disorder realization of the parity-block SYK OTOC or Green's
+    function: returns (t, v, e0, stats) with v_k / v_0 at t_k = k tmax/nsteps
+    and e0 the ground-state energy the Hamiltonian was shifted by."""
+    k = {"otoc": 0, "green": 1}[kind]
+    out = np.zeros(2 * (nsteps + 1))
+    e0 = _dbl()
+    st = EvolveStats()
+    a = None
+    if init is not None:
+        a = np.ascontiguousarray(init, np.complex64 if fp == 32 else np.complex128)
+        assert a.shape == (1 << (N // 2 - 1),)
+    hcheck(host().rph_syk_block_npoint(k, fp, N, sp, J, seed, int(regularize), otoc_idx,
+                                       None if a is None else _ptr(a), state_seed, beta, tmax,
+                                       nsteps, krylov_dim, tol, _ptr(out), _c.byref(e0),
+                                       _c.byref(st)))
+    t = np.arange(nsteps + 1) * (tmax / nsteps if nsteps else 0.0)
+    return t, out[0::2] + 1j * out[1::2], e0.value, st
+
+
+def format_result_lines(t, v, digits=6):
+    """Lines 't Re Im', one per time point, as the reference's result files
+    hold them (an ostream's default format: %g with 6 significant digits;
+    digits=17 keeps every bit of a double)."""
+    fmt = "%%.%dg" % digits
+    return ["%s %s %s" % (fmt % float(tk), fmt % float(np.real(vk)), fmt % float(np.imag(vk)))
+            for tk, vk in zip(t, v)]
+
+
+def write_result_file(path, t, v, digits=6):
+    with open(path, "w") as f:
+        for line in format_result_lines(t, v, digits):
+            f.write(line + "\n")
+
+
+def read_result_file(path):
+    """(t, v) from a 't Re Im' result file."""
+    rows = np.loadtxt(path, ndmin=2)
+    return rows[:, 0], rows[:, 1] + 1j * rows[:, 2]
 
 
 # ---------------------------------------------------------------------------

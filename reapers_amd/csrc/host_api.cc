@@ -678,3 +678,314 @@ int rph_state_ground(void *sp, void *hp, int krydim, double eps, int maxiter, do
 }
 
 }  // extern "C"
+
+// parity-block form ---------------------------------------------------------
+
+namespace {
+
+struct BlkSt {
+    int fp;
+    BlockState<float, GPUImpl> *f = nullptr;
+    BlockState<double, GPUImpl> *d = nullptr;
+    ~BlkSt() { delete f; delete d; }
+};
+
+template<RealScalar Fp>
+SumOps<Fp> &ham_of(Ham *h);
+template<>
+SumOps<float> &ham_of<float>(Ham *h) { return *h->f; }
+template<>
+SumOps<double> &ham_of<double>(Ham *h) { return *h->d; }
+
+template<RealScalar Fp>
+BlockState<Fp, GPUImpl> &blk_of(BlkSt *s);
+template<>
+BlockState<float, GPUImpl> &blk_of<float>(BlkSt *s) { return *s->f; }
+template<>
+BlockState<double, GPUImpl> &blk_of<double>(BlkSt *s) { return *s->d; }
+
+void check_fp(int fp, std::initializer_list<Ham *> hs)
+{
+    for (Ham *h : hs)
+        if (h && h->fp != fp) ThrowException(InvalidArgument, "precisions of state and operators must match");
+}
+
+// a block of a block operator: the handle's term list, or a null block
+template<RealScalar Fp>
+SumOps<Fp> block_of(Ham *h) { return h ? ham_of<Fp>(h) : SumOps<Fp>{}; }
+
+template<RealScalar Fp>
+void block_apply(BlkSt *s, Ham *LL, Ham *LR, Ham *RL, Ham *RR)
+{
+    auto &st = blk_of<Fp>(s);
+    if (!LR && !RL) {
+        st *= BlockDiag<SumOps<Fp>>(block_of<Fp>(LL), block_of<Fp>(RR));
+    } else if (!LL && !RR) {
+        st *= BlockAntiDiag<SumOps<Fp>>(block_of<Fp>(LR), block_of<Fp>(RL));
+    } else {
+        st *= BlockOp<SumOps<Fp>>(block_of<Fp>(LL), block_of<Fp>(LR), block_of<Fp>(RL),
+                                  block_of<Fp>(RR));
+    }
+}
+
+template<RealScalar Fp>
+void syk_block_npoint(int kind, int N, double sp, double J, uint32_t seed, int regularize,
+                      int otoc_idx, const void *init, uint64_t state_seed, double beta,
+                      double tmax, int nsteps, int krylov_dim, double tol, double *out,
+                      double *e0, KrylovStats &st)
+{
+    SYKBlockForm<Fp, GPUImpl> syk(N, (Fp)sp, (Fp)J);
+    std::mt19937 rg(seed);
+    auto ham = syk.gen_ham(rg, regularize != 0);
+    const auto &ops = syk.fermion_ops();
+    auto op = ops[N - 2].LR;
+    if (otoc_idx >= 0 && otoc_idx <= N - 2) op = ops[otoc_idx].LR;
+    State<Fp, GPUImpl> s0(N / 2 - 1);
+    if (init) s0.load((const complex<Fp> *)init);
+    else s0.random_state(state_seed);
+    KrylovOptions opt;
+    opt.krydim = krylov_dim;
+    opt.tol = tol;
+    auto v = npoint_block<Fp, GPUImpl>(kind ? NPoint::Green : NPoint::Otoc, ham, op, s0, tmax,
+                                       nsteps, beta, krylov_dim, opt, st, e0);
+    for (size_t k = 0; k < v.size(); k++) {
+        out[2 * k] = v[k].real();
+        out[2 * k + 1] = v[k].imag();
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+// SYKBlockForm::gen_ham(std::mt19937(seed), regularize): which = 0 exports
+// ham.LL, 1 ham.RR
+int rph_syk_block_gen(int fp, int N, double sp, double J, int std_gamma, uint32_t seed,
+                      int regularize, int which, uint64_t *bits, double *coeff, int64_t cap,
+                      int64_t *nterms)
+{
+    return guarded([&] {
+        std::mt19937 rg(seed);
+        if (fp == 32) {
+            SYKBlockForm<float, GPUImpl> syk(N, (float)sp, (float)J, std_gamma != 0);
+            auto ham = syk.gen_ham(rg, regularize != 0);
+            *nterms = export_ham<float>(which ? ham.RR : ham.LL, bits, coeff, cap);
+        } else {
+            SYKBlockForm<double, GPUImpl> syk(N, sp, J, std_gamma != 0);
+            auto ham = syk.gen_ham(rg, regularize != 0);
+            *nterms = export_ham<double>(which ? ham.RR : ham.LL, bits, coeff, cap);
+        }
+    });
+}
+
+// FermionOpBlockForm(N, n, std_gamma): which = 0 exports the LR block, 1 RL
+int rph_majorana_block(int fp, int N, int n, int std_gamma, int which, uint64_t *bits,
+                       double *coeff, int64_t cap, int64_t *nterms)
+{
+    return guarded([&] {
+        if (fp == 32) {
+            FermionOpBlockForm<float, GPUImpl> op(N, n, std_gamma != 0);
+            *nterms = export_ham<float>(which ? op.RL : op.LR, bits, coeff, cap);
+        } else {
+            FermionOpBlockForm<double, GPUImpl> op(N, n, std_gamma != 0);
+            *nterms = export_ham<double>(which ? op.RL : op.LR, bits, coeff, cap);
+        }
+    });
+}
+
+// chi_i chi_j + chi_j chi_i in block form (a BlockDiag): which = 0 exports
+// LL, 1 RR; *null receives the block's null flag
+int rph_block_anticomm(int fp, int N, int i, int j, int std_gamma, int which, uint64_t *bits,
+                       double *coeff, int64_t cap, int64_t *nterms, int *null)
+{
+    return guarded([&] {
+        auto run = [&](auto tag) {
+            using Fp = decltype(tag);
+            FermionOpBlockForm<Fp, GPUImpl> a(N, i, std_gamma != 0), b(N, j, std_gamma != 0);
+            auto c = a * b + b * a;
+            *null = which ? c.nullRR : c.nullLL;
+            *nterms = export_ham<Fp>(which ? c.RR : c.LL, bits, coeff, cap);
+        };
+        if (fp == 32) run(float{});
+        else run(double{});
+    });
+}
+
+void *rph_block_state_new(int fp, int len)
+{
+    BlkSt *s = nullptr;
+    int st = guarded([&] {
+        s = new BlkSt{fp};
+        if (fp == 32) s->f = new BlockState<float, GPUImpl>(len);
+        else s->d = new BlockState<double, GPUImpl>(len);
+    });
+    if (st) { delete s; return nullptr; }
+    return s;
+}
+
+void rph_block_state_free(void *sp) { delete (BlkSt *)sp; }
+
+// load the two sectors from host vectors of 2^len amplitudes; a null
+// pointer makes that sector null
+int rph_block_state_load(void *sp, const void *L, const void *R)
+{
+    return guarded([&] {
+        BlkSt *s = (BlkSt *)sp;
+        auto run = [&](auto tag) {
+            using Fp = decltype(tag);
+            auto &st = blk_of<Fp>(s);
+            int len = st.spin_chain_length();
+            for (int k = 0; k < 2; k++) {
+                const void *src = k ? R : L;
+                auto &v = k ? st.R : st.L;
+                bool &null = k ? st.nullR : st.nullL;
+                if (src) {
+                    if (!v.size()) v = State<Fp, GPUImpl>(len);
+                    v.load((const complex<Fp> *)src);
+                    null = false;
+                } else {
+                    v = State<Fp, GPUImpl>{};
+                    null = true;
+                }
+            }
+        };
+        if (s->fp == 32) run(float{});
+        else run(double{});
+    });
+}
+
+// store the defined sectors; *nullL / *nullR receive the flags
+int rph_block_state_store(void *sp, void *L, void *R, int *nullL, int *nullR)
+{
+    return guarded([&] {
+        BlkSt *s = (BlkSt *)sp;
+        auto run = [&](auto tag) {
+            using Fp = decltype(tag);
+            auto &st = blk_of<Fp>(s);
+            *nullL = st.nullL;
+            *nullR = st.nullR;
+            if (!st.nullL && L) st.L.store((complex<Fp> *)L);
+            if (!st.nullR && R) st.R.store((complex<Fp> *)R);
+        };
+        if (s->fp == 32) run(float{});
+        else run(double{});
+    });
+}
+
+int rph_block_state_random(void *sp, uint64_t seed)
+{
+    return guarded([&] {
+        BlkSt *s = (BlkSt *)sp;
+        if (s->fp == 32) s->f->random_state(seed);
+        else s->d->random_state(seed);
+    });
+}
+
+// amplitude i of the full vector (L sector first)
+int rph_block_state_get(void *sp, uint64_t i, double *re, double *im)
+{
+    return guarded([&] {
+        BlkSt *s = (BlkSt *)sp;
+        auto c = s->fp == 32 ? std::complex<double>((*s->f)[i]) : (*s->d)[i];
+        *re = c.real();
+        *im = c.imag();
+    });
+}
+
+int rph_block_state_norm(void *sp, double *nrm)
+{
+    return guarded([&] {
+        BlkSt *s = (BlkSt *)sp;
+        *nrm = s->fp == 32 ? s->f->norm64() : s->d->norm64();
+    });
+}
+
+// st *= op for the block operator with blocks LL, LR, RL, RR (null handles
+// are null blocks): BlockDiag when LR and RL are null, BlockAntiDiag when LL
+// and RR are, BlockOp otherwise
+int rph_block_state_apply(void *sp, void *LL, void *LR, void *RL, void *RR)
+{
+    return guarded([&] {
+        BlkSt *s = (BlkSt *)sp;
+        check_fp(s->fp, {(Ham *)LL, (Ham *)LR, (Ham *)RL, (Ham *)RR});
+        if (s->fp == 32) block_apply<float>(s, (Ham *)LL, (Ham *)LR, (Ham *)RL, (Ham *)RR);
+        else block_apply<double>(s, (Ham *)LL, (Ham *)LR, (Ham *)RL, (Ham *)RR);
+    });
+}
+
+// BlockState::evolve with BlockDiag(LL, RR)
+int rph_block_state_evolve(void *sp, void *LL, void *RR, double t, double beta, int krydim,
+                           double tol, rph_stats *out)
+{
+    return guarded([&] {
+        BlkSt *s = (BlkSt *)sp;
+        check_fp(s->fp, {(Ham *)LL, (Ham *)RR});
+        KrylovOptions opt;
+        opt.krydim = krydim;
+        opt.tol = tol;
+        auto t0 = std::chrono::steady_clock::now();
+        KrylovStats st;
+        auto run = [&](auto tag) {
+            using Fp = decltype(tag);
+            BlockDiag<SumOps<Fp>> h(block_of<Fp>((Ham *)LL), block_of<Fp>((Ham *)RR));
+            st = blk_of<Fp>(s).evolve_ex(h, t, beta, opt);
+        };
+        if (s->fp == 32) run(float{});
+        else run(double{});
+        GPUImpl::synchronize();
+        double wall = std::chrono::duration<double, std::milli>(
+                          std::chrono::steady_clock::now() - t0).count();
+        fill_stats(out, st, wall);
+    });
+}
+
+// BlockState::ground_state with BlockDiag(LL, RR); *sector = 0 if the L
+// sector was kept, 1 if R
+int rph_block_state_ground(void *sp, void *LL, void *RR, int krydim, double eps, int maxiter,
+                           double *energy, int *sector)
+{
+    return guarded([&] {
+        BlkSt *s = (BlkSt *)sp;
+        check_fp(s->fp, {(Ham *)LL, (Ham *)RR});
+        auto run = [&](auto tag) {
+            using Fp = decltype(tag);
+            BlockDiag<SumOps<Fp>> h(block_of<Fp>((Ham *)LL), block_of<Fp>((Ham *)RR));
+            auto &st = blk_of<Fp>(s);
+            *energy = st.ground_state(h, krydim, (Fp)eps, maxiter);
+            *sector = st.nullL ? 1 : 0;
+        };
+        if (s->fp == 32) run(float{});
+        else run(double{});
+    });
+}
+
+// One disorder realization of the reference's parity-block SYK driver:
+// SYKBlockForm(N, sp, J).gen_ham(std::mt19937(seed), regularize), the
+// initial L-sector state from init (2^(N/2-1) amplitudes) or random with
+// state_seed, then npoint_block (kind 0 = OTOC, 1 = Green's function).
+// out receives 2 (nsteps + 1) doubles (Re, Im of v_k / v_0), e0 the
+// ground-state energy the Hamiltonian was shifted by.
+int rph_syk_block_npoint(int kind, int fp, int N, double sp, double J, uint32_t seed,
+                         int regularize, int otoc_idx, const void *init, uint64_t state_seed,
+                         double beta, double tmax, int nsteps, int krylov_dim, double tol,
+                         double *out, double *e0, rph_stats *stats)
+{
+    return guarded([&] {
+        if (kind != 0 && kind != 1) ThrowException(InvalidArgument, "kind", "must be 0 or 1");
+        if (!out || nsteps < 0) ThrowException(InvalidArgument, "out", "null or negative count");
+        KrylovStats st;
+        auto t0 = std::chrono::steady_clock::now();
+        if (fp == 32)
+            syk_block_npoint<float>(kind, N, sp, J, seed, regularize, otoc_idx, init, state_seed,
+                                    beta, tmax, nsteps, krylov_dim, tol, out, e0, st);
+        else
+            syk_block_npoint<double>(kind, N, sp, J, seed, regularize, otoc_idx, init, state_seed,
+                                     beta, tmax, nsteps, krylov_dim, tol, out, e0, st);
+        GPUImpl::synchronize();
+        double wall = std::chrono::duration<double, std::milli>(
+                          std::chrono::steady_clock::now() - t0).count();
+        fill_stats(stats, st, wall);
+    });
+}
+
+}  // extern "C"

@@ -145,5 +145,53 @@ public:
     using BaseType::operator==;
 };
 
+// Majorana chi_n in parity-block form (the reference's FermionOpBlockForm,
+// inc/fermions.h:308-368).  Every chi_n anticommutes with the parity of the
+// N/2 sites, so it is block anti-diagonal between the two parity sectors of
+// 2^(N/2-1) amplitudes each, and its blocks are operators on N/2-1 sites:
+//     n <= N-2:  LR = RL = chi'_n, the Majorana of N-2 fermions without the
+//                1/sqrt(2) (for n = N-2 that is sigma_z(N/2-2));
+//     n == N-1:  LR = -i 1, RL = +i 1;
+// all divided by sqrt(2) for the standard normalization, as a product with
+// 1/sqrt(2.0) rounded to FpType.  A product of two is block diagonal.
+template<RealScalar FpType = DefFpType, typename Impl = DefImpl>
+class FermionOpBlockForm
+    : public BlockAntiDiag<SubspaceView<typename Impl::template SumOps<FpType>>> {
+    using BlockType = SubspaceView<typename Impl::template SumOps<FpType>>;
+    using BaseType = BlockAntiDiag<BlockType>;
+    using IndexType = u8;
+    IndexType N, n;
+
+public:
+    using ProductType = BlockDiag<BlockType>;
+
+    FermionOpBlockForm(IndexType N, IndexType n, bool standard_gamma = true)
+        : BaseType(BlockType(N / 2 - 1, SpinOp<FpType>::identity()),
+                   BlockType(N / 2 - 1, SpinOp<FpType>::identity())),
+          N(N), n(n) {
+        // same checks as the non-block form; n == N and N == 2 are then
+        // rejected by the (N-2)-fermion operator below
+        if (N < 2) ThrowException(InvalidArgument, "N", "must be at least two");
+        if (N & 1) ThrowException(InvalidArgument, "N", "must be even");
+        if (n > N) ThrowException(FieldIndexTooLarge, n, N);
+        if (n == N - 1) {
+            this->LR *= complex<FpType>{0, -1};
+            this->RL *= complex<FpType>{0, 1};
+        } else {
+            this->LR = FermionOpNonBlockForm<FpType, Impl>(N - 2, n, false);
+            this->RL = this->LR;
+        }
+        if (standard_gamma) *this /= sqrt(2.0);
+    }
+
+    int num_fermions() const { return N; }
+    int index() const { return n; }
+};
+
+#ifdef REAPERS_USE_PARITY_BLOCKS
+template<RealScalar FpType = DefFpType, typename Impl = DefImpl>
+using FermionOp = FermionOpBlockForm<FpType, Impl>;
+#else
 template<RealScalar FpType = DefFpType, typename Impl = DefImpl>
 using FermionOp = FermionOpNonBlockForm<FpType, Impl>;
+#endif

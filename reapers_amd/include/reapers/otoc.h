@@ -19,26 +19,35 @@
 #error "Do not include this file directly. Include the master header file reapers.h"
 #endif
 
+namespace internal {
+inline void add_stats(KrylovStats &st, const KrylovStats &s) {
+    st.matvecs += s.matvecs;
+    st.restarts += s.restarts;
+    st.gs_passes += s.gs_passes;
+    st.breakdown |= s.breakdown;
+    st.substeps += s.substeps;
+    st.rejected += s.rejected;
+    st.host_ms += s.host_ms;
+    st.last_nrm = s.last_nrm;
+}
+}  // namespace internal
+
+// The four-evolution OTOC with the forward (+t) evolutions under fwd and the
+// backward (-t) ones under bwd; the initial state is pre-evolved under bwd.
+// fwd == bwd is the single-Hamiltonian form; in parity-block form fwd is
+// the R block, bwd the L block, and A = B is a Majorana's LR block, which
+// carries the state from one sector to the other (examples/syk.cc:270-284).
 template<RealScalar FpType, typename Impl>
-std::vector<std::complex<double>> otoc(const typename Impl::template SumOps<FpType> &ham,
-                                       const typename Impl::template SumOps<FpType> &A,
-                                       const typename Impl::template SumOps<FpType> &B,
-                                       State<FpType, Impl> &init,
-                                       const std::vector<double> &times, double beta,
-                                       const KrylovOptions &opt, KrylovStats &st)
+std::vector<std::complex<double>> otoc2(const typename Impl::template SumOps<FpType> &fwd,
+                                        const typename Impl::template SumOps<FpType> &bwd,
+                                        const typename Impl::template SumOps<FpType> &A,
+                                        const typename Impl::template SumOps<FpType> &B,
+                                        State<FpType, Impl> &init,
+                                        const std::vector<double> &times, double beta,
+                                        const KrylovOptions &opt, KrylovStats &st)
 {
-    auto add = [&st](const KrylovStats &s) {
-        st.matvecs += s.matvecs;
-        st.restarts += s.restarts;
-        st.gs_passes += s.gs_passes;
-        st.breakdown |= s.breakdown;
-        st.substeps += s.substeps;
-        st.rejected += s.rejected;
-        st.host_ms += s.host_ms;
-        st.last_nrm = s.last_nrm;
-    };
     if (beta != 0.0) {
-        add(init.evolve_ex(ham, 0.0, beta / 8, opt));
+        internal::add_stats(st, init.evolve_ex(bwd, 0.0, beta / 8, opt));
         init.gc();
     }
     std::vector<std::complex<double>> v;
@@ -47,15 +56,96 @@ std::vector<std::complex<double>> otoc(const typename Impl::template SumOps<FpTy
     for (double t : times) {
         psi = init;
         psi *= B;
-        add(psi.evolve_ex(ham, t, beta / 4, opt));
+        internal::add_stats(st, psi.evolve_ex(fwd, t, beta / 4, opt));
         psi *= A;
-        add(psi.evolve_ex(ham, -t, beta / 4, opt));
+        internal::add_stats(st, psi.evolve_ex(bwd, -t, beta / 4, opt));
         psi *= B;
-        add(psi.evolve_ex(ham, t, beta / 4, opt));
+        internal::add_stats(st, psi.evolve_ex(fwd, t, beta / 4, opt));
         psi *= A;
-        add(psi.evolve_ex(ham, -t, 0.0, opt));
+        internal::add_stats(st, psi.evolve_ex(bwd, -t, 0.0, opt));
         v.push_back(init.inner64(psi));
         st.matvecs += 4;   // the four operator applications
     }
+    return v;
+}
+
+template<RealScalar FpType, typename Impl>
+std::vector<std::complex<double>> otoc(const typename Impl::template SumOps<FpType> &ham,
+                                       const typename Impl::template SumOps<FpType> &A,
+                                       const typename Impl::template SumOps<FpType> &B,
+                                       State<FpType, Impl> &init,
+                                       const std::vector<double> &times, double beta,
+                                       const KrylovOptions &opt, KrylovStats &st)
+{
+    return otoc2<FpType, Impl>(ham, ham, A, B, init, times, beta, opt, st);
+}
+
+// The reference's finite-temperature n-point functions of one disorder
+// realization in parity-block form (BaseEval::eval with the OTOC and Green
+// evaluators, examples/syk.cc:120-284), starting from a state s0 of the L
+// sector:
+//   1. E0 = the Lanczos ground-state energy of ham.LL (krylov_dim vectors,
+//      eps 0.1, from a random state; examples/syk.cc:137-140);
+//   2. ham.LL and ham.RR are shifted by -E0 times the identity, so that the
+//      imaginary-time factors exp(-beta (H - E0) / k) stay of order one;
+//   3. s0 is pre-evolved: exp(-beta/8 H_L) for the OTOC, exp(-beta/2 H_L)
+//      for the Green's function;
+//   4. for t = k tmax / nsteps, k = 0..nsteps, s = s0 and
+//        OTOC:  s *= op; s = e^{-(it+b/4) H_R} s; s = e^{-(-it+b/4) H_L} s;
+//               s *= op; s = e^{-(it+b/4) H_R} s; s = e^{it H_L} s
+//        Green: s = e^{-it H_R} s; s = e^{it H_L} s
+//      and v_k = <s0|s>;
+//   5. every v_k is divided by v_0.
+// op is ops[otoc_idx].LR (ops[N-2].LR when otoc_idx is outside [0, N-2]);
+// the Green's function is that of the last Majorana, whose blocks are
+// multiples of the identity.  The shifted Hamiltonian is left in ham, as in
+// the reference; e0 receives E0.
+enum class NPoint { Otoc, Green };
+
+template<RealScalar FpType, typename Impl, typename HamOpTy>
+std::vector<std::complex<double>> npoint_block(NPoint kind, BlockDiag<HamOpTy> &ham,
+                                               const typename Impl::template SumOps<FpType> &op,
+                                               State<FpType, Impl> &s0, double tmax, int nsteps,
+                                               double beta, int krylov_dim,
+                                               const KrylovOptions &opt, KrylovStats &st,
+                                               double *e0 = nullptr)
+{
+    if (nsteps < 0) ThrowException(InvalidArgument, "nsteps", "must not be negative");
+    if (ham.nullLL || ham.nullRR)
+        ThrowException(InvalidArgument, "Hamiltonian", "must have both parity blocks");
+    FpType gnd;
+    {
+        State<FpType, Impl> g(s0.spin_chain_length());
+        gnd = g.ground_state(ham.LL, krylov_dim, (FpType)0.1);
+    }
+    if (e0) *e0 = (double)gnd;
+    ham.LL += (-gnd) * SpinOp<FpType>::identity();
+    ham.RR += (-gnd) * SpinOp<FpType>::identity();
+    double pre = kind == NPoint::Otoc ? beta / 8 : beta / 2;
+    if (pre != 0.0) internal::add_stats(st, s0.evolve_ex(ham.LL, 0.0, pre, opt));
+    s0.gc();
+    std::vector<std::complex<double>> v;
+    v.reserve(nsteps + 1);
+    double dt = nsteps ? tmax / nsteps : 0.0;
+    State<FpType, Impl> s;
+    for (int k = 0; k <= nsteps; k++) {
+        double t = k * dt;
+        s = s0;
+        if (kind == NPoint::Otoc) {
+            s *= op;
+            internal::add_stats(st, s.evolve_ex(ham.RR, t, beta / 4, opt));
+            internal::add_stats(st, s.evolve_ex(ham.LL, -t, beta / 4, opt));
+            s *= op;
+            internal::add_stats(st, s.evolve_ex(ham.RR, t, beta / 4, opt));
+            internal::add_stats(st, s.evolve_ex(ham.LL, -t, 0.0, opt));
+            st.matvecs += 2;
+        } else {
+            internal::add_stats(st, s.evolve_ex(ham.RR, t, 0.0, opt));
+            internal::add_stats(st, s.evolve_ex(ham.LL, -t, 0.0, opt));
+        }
+        v.push_back(s0.inner64(s));
+    }
+    std::complex<double> v0 = v[0];
+    for (auto &x : v) x /= v0;
     return v;
 }

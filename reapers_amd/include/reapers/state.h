@@ -317,3 +317,166 @@ public:
         curbuf = 0;
     }
 };
+
+// A state in parity-block form (the reference's BlockState, inc/state.h:
+// 447-584): the L and the R parity sector as two States of len sites each.
+// Amplitude i of the full 2^(len+1) vector is L[i] below 2^len and
+// R[i - 2^len] from there on, L first; a null sector reads as zeros.  The
+// sectors are independent problems on the device: evolve() runs each one's
+// restarted Krylov with its own block of the Hamiltonian, and a single
+// Majorana (BlockAntiDiag) moves amplitudes between them.
+template<RealScalar FpType = DefFpType, typename Impl = DefImpl>
+struct BlockState : BlockVec<State<FpType, Impl>> {
+    using StateType = State<FpType, Impl>;
+    using Base = BlockVec<StateType>;
+
+private:
+    using LenType = typename SpinOp<FpType>::IndexType;
+    LenType slen;
+
+    LenType len_of(const Base &b) const {
+        return b.L.size() ? b.L.spin_chain_length() : b.R.spin_chain_length();
+    }
+
+    // a null sector holds an empty State; give it storage again
+    void define_sectors() {
+        if (!this->L.size()) this->L = StateType(slen);
+        if (!this->R.size()) this->R = StateType(slen);
+        this->nullL = this->nullR = false;
+    }
+
+public:
+    // len is the length of each sector's spin chain (the full space has
+    // 2^(len+1) amplitudes)
+    BlockState(LenType len) : Base(StateType(len), StateType(len)), slen(len) {}
+    BlockState(const Base &b) : Base(b), slen(len_of(b)) {}
+    BlockState(Base &&b) : Base(std::move(b)), slen(len_of(*this)) {}
+    BlockState(const BlockState &) = default;
+    BlockState(BlockState &&) = default;
+    BlockState &operator=(const Base &b) {
+        Base::operator=(b);
+        if (len_of(b)) slen = len_of(b);
+        return *this;
+    }
+    BlockState &operator=(Base &&b) {
+        LenType l = len_of(b);
+        Base::operator=(std::move(b));
+        if (l) slen = l;
+        return *this;
+    }
+    BlockState &operator=(const BlockState &b) {
+        Base::operator=(b);
+        slen = b.slen;
+        return *this;
+    }
+    BlockState &operator=(BlockState &&b) {
+        Base::operator=(std::move(b));
+        slen = b.slen;
+        return *this;
+    }
+
+    // each sector's own random vector, then the pair normalized together;
+    // null sectors become defined again
+    void random_state(u64 seed = 0) {
+        define_sectors();
+        this->L.random_state(seed);
+        this->R.random_state(seed ? seed ^ 0x9e3779b97f4a7c15ull : 0);
+        normalize();
+    }
+
+    // zero vectors in both sectors, which are then defined (not null)
+    void zero_state() {
+        define_sectors();
+        this->L.zero_state();
+        this->R.zero_state();
+    }
+
+    LenType spin_chain_length() const { return slen; }
+
+    size_t dim() const { return (size_t)1 << (spin_chain_length() + 1); }
+
+    complex<FpType> operator[](size_t i) const {
+        size_t half = dim() / 2;
+        if (i < half) return this->nullL ? complex<FpType>{} : this->L[i];
+        return this->nullR ? complex<FpType>{} : this->R[i - half];
+    }
+
+    double norm64() const {
+        double l = this->nullL ? 0.0 : this->L.norm64();
+        double r = this->nullR ? 0.0 : this->R.norm64();
+        return std::sqrt(l * l + r * r);
+    }
+
+    FpType norm() const { return (FpType)norm64(); }
+
+    void normalize() {
+        if (this->nullL && this->nullR) return;
+        double n = norm64();
+        if (!this->nullL) this->L *= complex<FpType>((FpType)(1.0 / n));
+        if (!this->nullR) this->R *= complex<FpType>((FpType)(1.0 / n));
+    }
+
+    // <this|other>, accumulated in FP64 per sector
+    std::complex<double> inner64(const BlockState &other) const {
+        std::complex<double> s{};
+        if (!this->nullL && !other.nullL) s += this->L.inner64(other.L);
+        if (!this->nullR && !other.nullR) s += this->R.inner64(other.R);
+        return s;
+    }
+
+    // Lanczos in each sector; the state keeps the sector with the lower
+    // energy and the other becomes null (inc/state.h:487-505)
+    template<template<typename> typename Algo = DefEvolutionAlgorithm, typename HamOpTy,
+             typename... Args>
+    FpType ground_state(const BlockDiag<HamOpTy> &ham, Args &&...args) {
+        FpType e0 = this->L.template ground_state<Algo>(ham.LL, args...);
+        FpType e1 = this->R.template ground_state<Algo>(ham.RR, args...);
+        if (e0 <= e1) {
+            this->R = StateType{};
+            this->nullR = true;
+            this->nullL = false;
+            return e0;
+        }
+        this->L = StateType{};
+        this->nullL = true;
+        this->nullR = false;
+        return e1;
+    }
+
+    // |psi> = exp(-(it + beta) H) |psi>, each defined sector with its block
+    template<template<typename> typename Algo = DefEvolutionAlgorithm, RealScalar F1,
+             RealScalar F2, typename HamOpTy, typename... Args>
+    void evolve(const BlockDiag<HamOpTy> &ham, F1 t, F2 beta = 0.0, Args &&...args) {
+        if ((t == 0.0) && (beta == 0.0)) return;
+        if (!this->nullL) this->L.template evolve<Algo>(ham.LL, t, beta, args...);
+        if (!this->nullR) this->R.template evolve<Algo>(ham.RR, t, beta, args...);
+    }
+
+    // the same with options; statistics summed over the sectors
+    template<typename HamOpTy>
+    KrylovStats evolve_ex(const BlockDiag<HamOpTy> &ham, double t, double beta,
+                          const KrylovOptions &opt) {
+        KrylovStats st;
+        for (int s = 0; s < 2; s++) {
+            bool null = s ? this->nullR : this->nullL;
+            if (null) continue;
+            StateType &v = s ? this->R : this->L;
+            const HamOpTy &h = s ? ham.RR : ham.LL;
+            KrylovStats k = v.evolve_ex(h, t, beta, opt);
+            st.matvecs += k.matvecs;
+            st.restarts += k.restarts;
+            st.gs_passes += k.gs_passes;
+            st.breakdown |= k.breakdown;
+            st.substeps += k.substeps;
+            st.rejected += k.rejected;
+            st.host_ms += k.host_ms;
+            st.last_nrm = k.last_nrm;
+        }
+        return st;
+    }
+
+    void gc() {
+        if (!this->nullL) this->L.gc();
+        if (!this->nullR) this->R.gc();
+    }
+};
