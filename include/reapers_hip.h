@@ -181,6 +181,39 @@ int rp_gs_pass_fused(int prec, const void *const *V, int k, void *w, uint64_t n,
 int rp_lincomb(int prec, void *y, const void *const *V, int k, const double *c, uint64_t n);
 
 /* ---------------------------------------------------------------------- */
+/* dense matrices and exact diagonalization -- DevSumOps::MatrixType       */
+/* (inc/gpuimpl.h:139-300), get_matrix / get_eigensystem / _matexp        */
+/* (inc/gpuimpl.h:370-388, 1179-1263), krnl_eye / krnl_vec_exp /          */
+/* krnl_scale_vecs (src/krnl.cu:113-154), mat_mul / mat_tr                 */
+/* (src/krnl.cu:260-300).  Matrices are column-major, leading dimension =  */
+/* rows, complex of the given precision, in caller-owned device memory;    */
+/* n = 2^len <= 2^15.                                                       */
+/* ---------------------------------------------------------------------- */
+/* the dense form of a term list (bits and interleaved coefficients as for
+ * rp_terms_upload): mat[r + c n] = <r|H|c>, n = 2^len */
+int rp_dense_build(int prec, int len, const uint64_t *bits, const double *coeff, uint64_t nterms,
+                   void *mat);
+/* Hermitian eigensystem (cusolverDnXsyevd in get_eigensystem,
+ * inc/gpuimpl.h:1179-1233): the lower triangle of mat is read, mat is
+ * overwritten by the column eigenvectors and w (device, n reals of the
+ * precision) receives the eigenvalues in ascending order */
+int rp_dense_eigh(int prec, uint64_t n, void *mat, void *w);
+/* exp(c H) = V diag(exp(c w)) V^H from H's eigensystem (_matexp,
+ * inc/gpuimpl.h:1235-1252); work is n*n scratch elements; res, work and V
+ * must be distinct */
+int rp_dense_expm_eig(int prec, uint64_t n, const void *V, const void *w, double c_re,
+                      double c_im, void *res, void *work);
+/* C = alpha op(A) op(B) + beta C with op 0 = as is, 1 = conjugate transpose
+ * (mat_mul, inc/gpuimpl.h:1024-1036); m x k times k x n */
+int rp_dense_gemm(int prec, uint64_t m, uint64_t n, uint64_t k, int op_a, int op_b, double a_re,
+                  double a_im, const void *A, uint64_t lda, const void *B, uint64_t ldb,
+                  double b_re, double b_im, void *C, uint64_t ldc);
+/* trace summed in FP64 (mat_tr, inc/gpuimpl.h:1040-1042) */
+int rp_dense_trace(int prec, uint64_t n, const void *mat, double *re, double *im);
+/* identity (krnl_eye via MatrixType::Identity, inc/gpuimpl.h:1107-1113) */
+int rp_dense_eye(int prec, uint64_t n, void *mat);
+
+/* ---------------------------------------------------------------------- */
 /* kernel timing (HIP events on the library stream)                       */
 /* ---------------------------------------------------------------------- */
 #define RP_KERNEL_APPLY 0           /* rp_apply_ops launches  */

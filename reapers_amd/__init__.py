@@ -194,6 +194,10 @@ def host():
                                       _c.c_int, _c.c_int, _vp, _u64, _dbl, _dbl, _c.c_int,
                                       _c.c_int, _dbl, _vp, _c.POINTER(_dbl),
                                       _c.POINTER(EvolveStats)], _c.c_int),
+            "rph_state_evolve_ed": ([_vp, _vp, _dbl, _dbl], _c.c_int),
+            "rph_state_ground_ed": ([_vp, _vp, _c.POINTER(_dbl)], _c.c_int),
+            "rph_ham_dense": ([_vp, _c.c_int, _dbl, _dbl, _vp, _vp], _c.c_int),
+            "rph_ham_matexp_entries": ([_vp, _c.POINTER(_i64)], _c.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(lib, name)
@@ -347,6 +351,33 @@ class Hamiltonian:
         hcheck(host().rph_ham_info(self._h, _c.byref(i)))
         return i
 
+    def _dense(self, what, c=0j, with_vals=False):
+        n = 1 << self.len
+        out = np.zeros(2 * n * n)
+        vals = np.zeros(n) if with_vals else None
+        hcheck(host().rph_ham_dense(self._h, what, c.real, c.imag, _ptr(out),
+                                    None if vals is None else _ptr(vals)))
+        m = (out[0::2] + 1j * out[1::2]).reshape(n, n).T  # column-major on the device
+        return m, vals
+
+    def matrix(self):
+        """The dense form built on the device (get_matrix)."""
+        return self._dense(0)[0]
+
+    def matexp(self, c):
+        """exp(c H) by exact diagonalization on the device (matexp, cached)."""
+        return self._dense(1, complex(c))[0]
+
+    def eigensystem(self):
+        """(eigenvalues ascending, column eigenvectors) from rocSOLVER heevd."""
+        m, vals = self._dense(2, with_vals=True)
+        return vals, m
+
+    def matexp_cache_entries(self):
+        n = _i64()
+        hcheck(host().rph_ham_matexp_entries(self._h, _c.byref(n)))
+        return n.value
+
     def close(self):
         if self._h:
             host().rph_ham_free(self._h)
@@ -439,6 +470,15 @@ class DeviceState:
     def ground_state(self, ham, krydim=5, eps=1e-10, maxiter=0):
         e = _dbl()
         hcheck(host().rph_state_ground(self._s, ham._h, krydim, eps, maxiter, _c.byref(e)))
+        return e.value
+
+    def evolve_ed(self, ham, t, beta=0.0):
+        """exp(-(beta + it) H) by exact diagonalization (ExactDiagonalization)."""
+        hcheck(host().rph_state_evolve_ed(self._s, ham._h, t, beta))
+
+    def ground_state_ed(self, ham):
+        e = _dbl()
+        hcheck(host().rph_state_ground_ed(self._s, ham._h, _c.byref(e)))
         return e.value
 
     def close(self):
@@ -540,8 +580,10 @@ def syk_block_npoint(N, kind="otoc", sp=0.0, J=1.0, seed=0, regularize=False, ot
                      tol=0.0, fp=32):
     """One disorder realization of the parity-block SYK OTOC or Green's
     function: returns (t, v, e0, stats) with v_k / v_0 at t_k = k tmax/nsteps
-    and e0 the ground-state energy the Hamiltonian was shifted by."""
-    k = {"otoc": 0, "green": 1}[kind]
+    and e0 the ground-state energy the Hamiltonian was shifted by.  kind
+    'otoc-trace' / 'green-trace' use the trace definitions by exact
+    diagonalization (no state; e0 = 0)."""
+    k = {"otoc": 0, "green": 1, "otoc-trace": 2, "green-trace": 3}[kind]
     out = np.zeros(2 * (nsteps + 1))
     e0 = _dbl()
     st = EvolveStats()

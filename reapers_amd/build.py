@@ -16,7 +16,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 
-HIP_SOURCES = ["rp_runtime.hip", "rp_vec.hip", "rp_apply.hip", "rp_mlp.hip", "rp_batch.hip"]
+HIP_SOURCES = ["rp_runtime.hip", "rp_vec.hip", "rp_apply.hip", "rp_mlp.hip", "rp_batch.hip",
+               "rp_dense.hip"]
 HOST_SOURCES = ["host_api.cc"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -70,7 +71,8 @@ def build_hip(force=False):
     lib = os.path.join(HERE, "libreapers_hip.so")
     if force or _stale(lib, objs):
         tmp = lib + ".tmp"
-        _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
+        _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
+             + ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"])
         os.replace(tmp, lib)
     return lib
 

@@ -740,6 +740,16 @@ void syk_block_npoint(int kind, int N, double sp, double J, uint32_t seed, int r
     const auto &ops = syk.fermion_ops();
     auto op = ops[N - 2].LR;
     if (otoc_idx >= 0 && otoc_idx <= N - 2) op = ops[otoc_idx].LR;
+    if (kind >= 2) {
+        auto v = npoint_block_trace<Fp>(kind == 2 ? NPoint::Otoc : NPoint::Green, ham, op, tmax,
+                                        nsteps, beta);
+        for (size_t k = 0; k < v.size(); k++) {
+            out[2 * k] = v[k].real();
+            out[2 * k + 1] = v[k].imag();
+        }
+        if (e0) *e0 = 0.0;
+        return;
+    }
     State<Fp, GPUImpl> s0(N / 2 - 1);
     if (init) s0.load((const complex<Fp> *)init);
     else s0.random_state(state_seed);
@@ -962,7 +972,9 @@ int rph_block_state_ground(void *sp, void *LL, void *RR, int krydim, double eps,
 // One disorder realization of the reference's parity-block SYK driver:
 // SYKBlockForm(N, sp, J).gen_ham(std::mt19937(seed), regularize), the
 // initial L-sector state from init (2^(N/2-1) amplitudes) or random with
-// state_seed, then npoint_block (kind 0 = OTOC, 1 = Green's function).
+// state_seed, then npoint_block (kind 0 = OTOC, 1 = Green's function), or
+// the trace definitions by exact diagonalization (kind 2 = OTOC, 3 = Green's
+// function; no state, e0 = 0).
 // out receives 2 (nsteps + 1) doubles (Re, Im of v_k / v_0), e0 the
 // ground-state energy the Hamiltonian was shifted by.
 int rph_syk_block_npoint(int kind, int fp, int N, double sp, double J, uint32_t seed,
@@ -971,7 +983,7 @@ int rph_syk_block_npoint(int kind, int fp, int N, double sp, double J, uint32_t 
                          double *out, double *e0, rph_stats *stats)
 {
     return guarded([&] {
-        if (kind != 0 && kind != 1) ThrowException(InvalidArgument, "kind", "must be 0 or 1");
+        if (kind < 0 || kind > 3) ThrowException(InvalidArgument, "kind", "must be 0..3");
         if (!out || nsteps < 0) ThrowException(InvalidArgument, "out", "null or negative count");
         KrylovStats st;
         auto t0 = std::chrono::steady_clock::now();
@@ -985,6 +997,80 @@ int rph_syk_block_npoint(int kind, int fp, int N, double sp, double J, uint32_t 
         double wall = std::chrono::duration<double, std::milli>(
                           std::chrono::steady_clock::now() - t0).count();
         fill_stats(stats, st, wall);
+    });
+}
+
+}  // extern "C"
+
+// exact diagonalization ------------------------------------------------------
+
+extern "C" {
+
+// State::evolve<ExactDiagonalization>(H, t, beta)
+int rph_state_evolve_ed(void *sp, void *hp, double t, double beta)
+{
+    return guarded([&] {
+        St *s = (St *)sp;
+        Ham *h = (Ham *)hp;
+        if (s->fp != h->fp) ThrowException(InvalidArgument, "precision of state and operator must match");
+        if (s->fp == 32) s->f->evolve<EvolutionAlgorithm::ExactDiagonalization>(*h->f, t, beta);
+        else s->d->evolve<EvolutionAlgorithm::ExactDiagonalization>(*h->d, t, beta);
+    });
+}
+
+// State::ground_state<ExactDiagonalization>(H)
+int rph_state_ground_ed(void *sp, void *hp, double *energy)
+{
+    return guarded([&] {
+        St *s = (St *)sp;
+        Ham *h = (Ham *)hp;
+        if (s->fp != h->fp) ThrowException(InvalidArgument, "precision of state and operator must match");
+        if (s->fp == 32)
+            *energy = s->f->ground_state<EvolutionAlgorithm::ExactDiagonalization>(*h->f);
+        else
+            *energy = s->d->ground_state<EvolutionAlgorithm::ExactDiagonalization>(*h->d);
+    });
+}
+
+// host copies of the handle's dense forms on its chain length: what = 0
+// the matrix, 1 exp(c H) (c = c_re + i c_im), 2 the eigenvectors; out holds
+// 4^len complex values (column-major, interleaved doubles); eigvals (2^len
+// doubles, may be null) receives the eigenvalues for what = 2
+int rph_ham_dense(void *hp, int what, double c_re, double c_im, double *out, double *eigvals)
+{
+    return guarded([&] {
+        Ham *h = (Ham *)hp;
+        auto run = [&](auto tag) {
+            using Fp = decltype(tag);
+            auto &ops = ham_of<Fp>(h);
+            const DevMatrix<Fp> *m = nullptr;
+            if (what == 0) m = &ops.get_matrix(h->len);
+            else if (what == 1) m = &ops.matexp(complex<Fp>{(Fp)c_re, (Fp)c_im}, h->len);
+            else {
+                auto es = ops.get_eigensystem(h->len);
+                if (eigvals)
+                    for (size_t i = 0; i < std::get<0>(es).size(); i++) eigvals[i] = std::get<0>(es)[i];
+                m = &std::get<1>(es);
+            }
+            auto v = m->to_host();
+            for (size_t i = 0; i < v.size(); i++) {
+                out[2 * i] = v[i].real();
+                out[2 * i + 1] = v[i].imag();
+            }
+        };
+        if (what < 0 || what > 2) ThrowException(InvalidArgument, "what", "must be 0..2");
+        if (h->fp == 32) run(float{});
+        else run(double{});
+    });
+}
+
+// entries in the handle's matexp cache
+int rph_ham_matexp_entries(void *hp, int64_t *n)
+{
+    return guarded([&] {
+        Ham *h = (Ham *)hp;
+        *n = h->fp == 32 ? (int64_t)h->f->matexp_cache_entries()
+                         : (int64_t)h->d->matexp_cache_entries();
     });
 }
 

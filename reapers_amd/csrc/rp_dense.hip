@@ -1,0 +1,366 @@
+// rp_dense.hip -- dense matrices and exact diagonalization: the dense form of
+// a term list, the Hermitian eigensystem, the matrix exponential from it,
+// products, traces (the reference's DevSumOps::MatrixType, get_matrix,
+// get_eigensystem and _matexp, inc/gpuimpl.h:139-300 and 1179-1263, with the
+// kernels krnl_vec_exp / krnl_scale_vecs / krnl_eye, src/krnl.cu:113-154).
+//
+// This is the small-N path (up to 2^15 amplitudes): a matrix of n x n
+// complex elements, column-major with leading dimension n, in caller-owned
+// device memory.  The O(n^3) parts -- the eigensolver and the products --
+// are rocSOLVER's heevd and rocBLAS's gemm on the library stream; the
+// library's own kernels build the matrix from the terms, scale the
+// eigenvector columns by exp(c w_j) (one pass, the exponential computed in
+// FP64 for both precisions) and reduce traces in FP64.
+#include "rp_internal.h"
+
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <vector>
+
+namespace rp {
+
+template <typename R> struct Cx2;
+template <> struct Cx2<float> { using T = float2; };
+template <> struct Cx2<double> { using T = double2; };
+
+constexpr int kDenseMaxLen = 15;
+constexpr int kDenseThreads = 256;
+
+// one thread per column c: H[c ^ x][c] = sum over the terms with flip mask x
+// of coeff (-1)^popcount(c & z) -- the Pauli string maps |c> to
+// (-1)^popcount(c & z) |c ^ x> up to the coefficient (inc/ops.h:421-440).
+// Terms arrive sorted by x, so each element is written once, summed in FP64.
+template <typename R>
+__global__ void __launch_bounds__(kDenseThreads)
+k_dense_build(typename Cx2<R>::T *__restrict__ mat, uint64_t n, const uint64_t *__restrict__ xs,
+              const uint64_t *__restrict__ zs, const double *__restrict__ co, uint64_t nterms)
+{
+    uint64_t c = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    typename Cx2<R>::T *col = mat + c * n;
+    uint64_t j = 0;
+    while (j < nterms) {
+        uint64_t x = xs[j];
+        double sr = 0.0, si = 0.0;
+        for (; j < nterms && xs[j] == x; j++) {
+            double s = (__popcll(c & zs[j]) & 1) ? -1.0 : 1.0;
+            sr += s * co[2 * j];
+            si += s * co[2 * j + 1];
+        }
+        typename Cx2<R>::T v;
+        v.x = (R)sr;
+        v.y = (R)si;
+        col[c ^ x] = v;
+    }
+}
+
+// tmp[:, j] = V[:, j] * exp(c w_j); grid.y = column
+template <typename R>
+__global__ void __launch_bounds__(kDenseThreads)
+k_scale_cols_exp(typename Cx2<R>::T *__restrict__ out, const typename Cx2<R>::T *__restrict__ V,
+                 const R *__restrict__ w, uint64_t n, double cr, double ci)
+{
+    uint64_t j = blockIdx.y;
+    double wj = (double)w[j];
+    double mag = exp(cr * wj);
+    double er = mag * cos(ci * wj), ei = mag * sin(ci * wj);
+    R fr = (R)er, fi = (R)ei;
+    const typename Cx2<R>::T *src = V + j * n;
+    typename Cx2<R>::T *dst = out + j * n;
+    for (uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; r < n;
+         r += (uint64_t)gridDim.x * blockDim.x) {
+        typename Cx2<R>::T v = src[r], o;
+        o.x = v.x * fr - v.y * fi;
+        o.y = v.x * fi + v.y * fr;
+        dst[r] = o;
+    }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(kDenseThreads)
+k_eye(typename Cx2<R>::T *__restrict__ mat, uint64_t n)
+{
+    uint64_t total = n * n;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        typename Cx2<R>::T v;
+        v.x = (i % (n + 1) == 0) ? (R)1 : (R)0;
+        v.y = (R)0;
+        mat[i] = v;
+    }
+}
+
+// one workgroup: the diagonal summed in FP64 (fixed order per n)
+template <typename R>
+__global__ void __launch_bounds__(kDenseThreads)
+k_trace(const typename Cx2<R>::T *__restrict__ mat, uint64_t n, double *__restrict__ out)
+{
+    __shared__ double sr[kDenseThreads], si[kDenseThreads];
+    double ar = 0.0, ai = 0.0;
+    for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) {
+        typename Cx2<R>::T v = mat[i * (n + 1)];
+        ar += (double)v.x;
+        ai += (double)v.y;
+    }
+    sr[threadIdx.x] = ar;
+    si[threadIdx.x] = ai;
+    __syncthreads();
+    for (int s = kDenseThreads / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            sr[threadIdx.x] += sr[threadIdx.x + s];
+            si[threadIdx.x] += si[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = sr[0];
+        out[1] = si[0];
+    }
+}
+
+// the rocBLAS handle of the library stream (created on first use; the
+// stream is re-bound on every call since rp_shutdown/rp_init recreate it)
+static rocblas_handle blas()
+{
+    static rocblas_handle h = nullptr;
+    if (!h && rocblas_create_handle(&h) != rocblas_status_success) h = nullptr;
+    if (h) rocblas_set_stream(h, ctx().stream);
+    return h;
+}
+
+static int blas_error(const char *what, rocblas_status s)
+{
+    return set_error(RP_ERR_RUNTIME, "%s failed: %s", what, rocblas_status_to_string(s));
+}
+
+static inline int grid_of(uint64_t units)
+{
+    uint64_t g = (units + kDenseThreads - 1) / kDenseThreads;
+    return (int)std::min<uint64_t>(std::max<uint64_t>(g, 1), 65536);
+}
+
+#define RP_DENSE_LAUNCH(kernel, grid, block, ...)                                 \
+    do {                                                                          \
+        hipLaunchKernelGGL(kernel, (grid), (block), 0, ::rp::ctx().stream, __VA_ARGS__); \
+        RP_CHECK_LAUNCH();                                                        \
+    } while (0)
+
+static bool dense_size_ok(uint64_t n)
+{
+    return n >= 1 && n <= (1ull << kDenseMaxLen);
+}
+
+}  // namespace rp
+
+using namespace rp;
+
+extern "C" {
+
+int rp_dense_build(int prec, int len, const uint64_t *bits, const double *coeff, uint64_t nterms,
+                   void *mat)
+{
+    int st = check_init();
+    if (st) return st;
+    RP_REQUIRE(valid_prec(prec), "invalid precision");
+    RP_REQUIRE(len >= 1 && len <= kDenseMaxLen, "len must be 1..15");
+    RP_REQUIRE(mat != nullptr, "null matrix");
+    RP_REQUIRE(nterms == 0 || (bits && coeff), "null term arrays");
+    uint64_t n = 1ull << len;
+    uint64_t mask = n - 1;
+    // flip and phase masks; terms sorted by x (stable: equal x keep their order)
+    std::vector<uint64_t> x(nterms), z(nterms);
+    for (uint64_t j = 0; j < nterms; j++) {
+        uint64_t b = bits[j], xm = 0, zm = 0;
+        for (int s = 0; s < 32 && b; s++, b >>= 2) {
+            xm |= (b & 1ull) << s;
+            zm |= ((b >> 1) & 1ull) << s;
+        }
+        RP_REQUIRE((xm & ~mask) == 0 && (zm & ~mask) == 0, "a term acts beyond len sites");
+        x[j] = xm;
+        z[j] = zm;
+    }
+    std::vector<uint64_t> order(nterms);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](uint64_t a, uint64_t b) { return x[a] < x[b]; });
+    std::vector<uint64_t> hx(nterms), hz(nterms);
+    std::vector<double> hc(2 * nterms);
+    for (uint64_t k = 0; k < nterms; k++) {
+        hx[k] = x[order[k]];
+        hz[k] = z[order[k]];
+        hc[2 * k] = coeff[2 * order[k]];
+        hc[2 * k + 1] = coeff[2 * order[k] + 1];
+    }
+    size_t ebytes = elem_bytes(prec);
+    Context &c = ctx();
+    RP_CHECK_HIP(hipMemsetAsync(mat, 0, n * n * ebytes, c.stream));
+    if (nterms == 0) return RP_OK;
+    uint64_t *dx = nullptr;
+    void *dbuf = nullptr;
+    size_t tbytes = nterms * (2 * sizeof(uint64_t) + 2 * sizeof(double));
+    RP_CHECK_HIP(hipMalloc(&dbuf, tbytes));
+    dx = (uint64_t *)dbuf;
+    uint64_t *dz = dx + nterms;
+    double *dc = (double *)(dz + nterms);
+    int rc = RP_OK;
+    do {
+        hipError_t e = hipMemcpyAsync(dx, hx.data(), nterms * 8, hipMemcpyHostToDevice, c.stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(dz, hz.data(), nterms * 8, hipMemcpyHostToDevice, c.stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(dc, hc.data(), nterms * 16, hipMemcpyHostToDevice, c.stream);
+        if (e != hipSuccess) {
+            rc = set_error(RP_ERR_RUNTIME, "rp_dense_build: copy failed: %s", hipGetErrorString(e));
+            break;
+        }
+        if (prec == RP_C64)
+            hipLaunchKernelGGL(k_dense_build<float>, dim3(grid_of(n)), dim3(kDenseThreads), 0,
+                               c.stream, (float2 *)mat, n, dx, dz, dc, nterms);
+        else
+            hipLaunchKernelGGL(k_dense_build<double>, dim3(grid_of(n)), dim3(kDenseThreads), 0,
+                               c.stream, (double2 *)mat, n, dx, dz, dc, nterms);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+        if (e != hipSuccess)
+            rc = set_error(RP_ERR_RUNTIME, "rp_dense_build: %s", hipGetErrorString(e));
+    } while (0);
+    (void)hipFree(dbuf);
+    return rc;
+}
+
+int rp_dense_eigh(int prec, uint64_t n, void *mat, void *w)
+{
+    int st = check_init();
+    if (st) return st;
+    RP_REQUIRE(valid_prec(prec), "invalid precision");
+    RP_REQUIRE(dense_size_ok(n), "n must be 1..32768");
+    RP_REQUIRE(mat && w, "null pointer");
+    rocblas_handle h = blas();
+    if (!h) return set_error(RP_ERR_RUNTIME, "rocblas_create_handle failed");
+    Context &c = ctx();
+    size_t rbytes = prec == RP_C64 ? sizeof(float) : sizeof(double);
+    void *work = nullptr;
+    RP_CHECK_HIP(hipMalloc(&work, n * rbytes + sizeof(rocblas_int)));
+    rocblas_int *dinfo = (rocblas_int *)((char *)work + n * rbytes);
+    rocblas_status s;
+    if (prec == RP_C64)
+        s = rocsolver_cheevd(h, rocblas_evect_original, rocblas_fill_lower, (rocblas_int)n,
+                             (rocblas_float_complex *)mat, (rocblas_int)n, (float *)w,
+                             (float *)work, dinfo);
+    else
+        s = rocsolver_zheevd(h, rocblas_evect_original, rocblas_fill_lower, (rocblas_int)n,
+                             (rocblas_double_complex *)mat, (rocblas_int)n, (double *)w,
+                             (double *)work, dinfo);
+    rocblas_int info = 0;
+    hipError_t e = hipSuccess;
+    if (s == rocblas_status_success) {
+        e = hipMemcpyAsync(&info, dinfo, sizeof(info), hipMemcpyDeviceToHost, c.stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+    }
+    (void)hipFree(work);
+    if (s != rocblas_status_success) return blas_error("rocsolver heevd", s);
+    if (e != hipSuccess) return set_error(RP_ERR_RUNTIME, "rp_dense_eigh: %s", hipGetErrorString(e));
+    if (info != 0) return set_error(RP_ERR_RUNTIME, "rocsolver heevd: info = %d", (int)info);
+    return RP_OK;
+}
+
+int rp_dense_expm_eig(int prec, uint64_t n, const void *V, const void *w, double c_re,
+                      double c_im, void *res, void *work)
+{
+    int st = check_init();
+    if (st) return st;
+    RP_REQUIRE(valid_prec(prec), "invalid precision");
+    RP_REQUIRE(dense_size_ok(n), "n must be 1..32768");
+    RP_REQUIRE(V && w && res && work, "null pointer");
+    RP_REQUIRE(res != V && res != work && work != V, "res, work and V must be distinct");
+    Context &c = ctx();
+    dim3 grid((unsigned)std::min<uint64_t>((n + kDenseThreads - 1) / kDenseThreads, 64),
+              (unsigned)n);
+    if (prec == RP_C64)
+        RP_DENSE_LAUNCH(k_scale_cols_exp<float>, grid, dim3(kDenseThreads),
+                  (float2 *)work, (const float2 *)V, (const float *)w, n, c_re, c_im);
+    else
+        RP_DENSE_LAUNCH(k_scale_cols_exp<double>, grid, dim3(kDenseThreads),
+                  (double2 *)work, (const double2 *)V, (const double *)w, n, c_re, c_im);
+    (void)c;
+    // res = work * V^H
+    return rp_dense_gemm(prec, n, n, n, 0, 1, 1.0, 0.0, work, n, V, n, 0.0, 0.0, res, n);
+}
+
+int rp_dense_gemm(int prec, uint64_t m, uint64_t n, uint64_t k, int op_a, int op_b, double a_re,
+                  double a_im, const void *A, uint64_t lda, const void *B, uint64_t ldb,
+                  double b_re, double b_im, void *C, uint64_t ldc)
+{
+    int st = check_init();
+    if (st) return st;
+    RP_REQUIRE(valid_prec(prec), "invalid precision");
+    RP_REQUIRE(m >= 1 && n >= 1 && k >= 1 && m < (1ull << 31) && n < (1ull << 31) &&
+               k < (1ull << 31), "dimensions must be 1..2^31-1");
+    RP_REQUIRE((op_a == 0 || op_a == 1) && (op_b == 0 || op_b == 1), "op must be 0 or 1");
+    RP_REQUIRE(A && B && C, "null pointer");
+    RP_REQUIRE(lda >= (op_a ? k : m) && ldb >= (op_b ? n : k) && ldc >= m, "leading dimension too small");
+    rocblas_handle h = blas();
+    if (!h) return set_error(RP_ERR_RUNTIME, "rocblas_create_handle failed");
+    rocblas_operation ta = op_a ? rocblas_operation_conjugate_transpose : rocblas_operation_none;
+    rocblas_operation tb = op_b ? rocblas_operation_conjugate_transpose : rocblas_operation_none;
+    rocblas_status s;
+    if (prec == RP_C64) {
+        rocblas_float_complex al{(float)a_re, (float)a_im}, be{(float)b_re, (float)b_im};
+        s = rocblas_cgemm(h, ta, tb, (rocblas_int)m, (rocblas_int)n, (rocblas_int)k, &al,
+                          (const rocblas_float_complex *)A, (rocblas_int)lda,
+                          (const rocblas_float_complex *)B, (rocblas_int)ldb, &be,
+                          (rocblas_float_complex *)C, (rocblas_int)ldc);
+    } else {
+        rocblas_double_complex al{a_re, a_im}, be{b_re, b_im};
+        s = rocblas_zgemm(h, ta, tb, (rocblas_int)m, (rocblas_int)n, (rocblas_int)k, &al,
+                          (const rocblas_double_complex *)A, (rocblas_int)lda,
+                          (const rocblas_double_complex *)B, (rocblas_int)ldb, &be,
+                          (rocblas_double_complex *)C, (rocblas_int)ldc);
+    }
+    if (s != rocblas_status_success) return blas_error("rocblas gemm", s);
+    return RP_OK;
+}
+
+int rp_dense_trace(int prec, uint64_t n, const void *mat, double *re, double *im)
+{
+    int st = check_init();
+    if (st) return st;
+    RP_REQUIRE(valid_prec(prec), "invalid precision");
+    RP_REQUIRE(dense_size_ok(n), "n must be 1..32768");
+    RP_REQUIRE(mat && re && im, "null pointer");
+    Context &c = ctx();
+    double *out = (double *)c.work;
+    if (prec == RP_C64)
+        RP_DENSE_LAUNCH(k_trace<float>, dim3(1), dim3(kDenseThreads),
+                  (const float2 *)mat, n, out);
+    else
+        RP_DENSE_LAUNCH(k_trace<double>, dim3(1), dim3(kDenseThreads),
+                  (const double2 *)mat, n, out);
+    RP_CHECK_HIP(hipMemcpyAsync(c.pinned, out, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                                c.stream));
+    RP_CHECK_HIP(hipStreamSynchronize(c.stream));
+    *re = c.pinned[0];
+    *im = c.pinned[1];
+    return RP_OK;
+}
+
+int rp_dense_eye(int prec, uint64_t n, void *mat)
+{
+    int st = check_init();
+    if (st) return st;
+    RP_REQUIRE(valid_prec(prec), "invalid precision");
+    RP_REQUIRE(dense_size_ok(n), "n must be 1..32768");
+    RP_REQUIRE(mat != nullptr, "null matrix");
+    if (prec == RP_C64)
+        RP_DENSE_LAUNCH(k_eye<float>, dim3(grid_of(n * n)), dim3(kDenseThreads),
+                  (float2 *)mat, n);
+    else
+        RP_DENSE_LAUNCH(k_eye<double>, dim3(grid_of(n * n)), dim3(kDenseThreads),
+                  (double2 *)mat, n);
+    return RP_OK;
+}
+
+}  // extern "C"

@@ -280,3 +280,34 @@ public:
         return (FpType)energy;
     }
 };
+
+// Evolution and ground state by exact diagonalization (the reference's
+// ExactDiagonalization, inc/algo.h:28-54): s = exp(-(beta + i t) H) s as one
+// product with the cached matrix exponential, and the ground state as the
+// eigenvector of the lowest eigenvalue (the first of the ascending list, as
+// minCoeff returns the first minimum).  For chains of up to 15 sites.
+template<typename Impl>
+class ExactDiagonalization {
+    template<RealScalar FpType>
+    using SumOps = typename Impl::template SumOps<FpType>;
+
+public:
+    template<RealScalar FpType>
+    static FpType ground_state(State<FpType, Impl> &s, const SumOps<FpType> &ham) {
+        auto len = s.spin_chain_length();
+        auto &&es = ham.get_eigensystem(len);
+        const auto &vals = std::get<0>(es);
+        size_t idx = std::min_element(vals.begin(), vals.end()) - vals.begin();
+        Impl::copy_column(s.dim(), s.buf(), std::get<1>(es), idx);
+        return vals[idx];
+    }
+
+    template<RealScalar FpType, typename... Args>
+    static void evolve(State<FpType, Impl> &s, const SumOps<FpType> &ops, FpType t,
+                       FpType beta = 0.0, Args &&...) {
+        s.enlarge(2);
+        const auto &m = ops.matexp(complex<FpType>{-beta, -t}, s.spin_chain_length());
+        Impl::mat_vec(s.dim(), s.buf(1), m, s.buf());
+        s.inc_curbuf(1);
+    }
+};

@@ -111,6 +111,18 @@ public:
     }
 
     int spin_chain_length() const { return len; }
+
+    // dense forms on this view's chain length (inc/fermions.h:183-203)
+    decltype(auto) get_matrix() const { return SumOpsTy::get_matrix(len); }
+    decltype(auto) get_eigensystem() const { return SumOpsTy::get_eigensystem(len); }
+    decltype(auto) matexp(typename SumOpsTy::RealScalarType c) const {
+        return SumOpsTy::matexp(typename SumOpsTy::ComplexScalarType(c), len);
+    }
+    decltype(auto) matexp(typename SumOpsTy::ComplexScalarType c) const {
+        return SumOpsTy::matexp(c, len);
+    }
+    using SumOpsTy::matexp;
+    using SumOpsTy::get_matrix;
 };
 
 template<RealScalar FpType = DefFpType, typename Impl = DefImpl>

@@ -27,13 +27,21 @@ template<RealScalar FpType> constexpr int rp_prec() {
 }
 }  // namespace internal
 
+template<RealScalar FpType> class DevMatrix;
+namespace internal {
+template<RealScalar FpType> struct DenseCache;
+}
+
 // A SumOps whose device layout is uploaded on first use and dropped on every
-// host mutation (DevSumOps::upload/release, inc/gpuimpl.h:1064-1105).
+// host mutation (DevSumOps::upload/release, inc/gpuimpl.h:1064-1105).  Its
+// dense forms (matrix, eigensystem, matrix exponentials; dense.h) are built
+// on request and dropped the same way.
 template<RealScalar FpType = DefFpType>
 class DevSumOps : public HostSumOps<FpType> {
     friend class GPUImpl;
     mutable rp_terms_t handle = nullptr;
     mutable int handle_len = -1;
+    mutable std::unique_ptr<internal::DenseCache<FpType>> dense;
 
     void release() const override {
         if (handle) {
@@ -41,7 +49,11 @@ class DevSumOps : public HostSumOps<FpType> {
             handle = nullptr;
             handle_len = -1;
         }
+        dense.reset();
     }
+
+    internal::DenseCache<FpType> &dense_at(int len) const;
+    DevMatrix<FpType> _matexp(complex<FpType> c, int len) const;
 
 public:
     using HostSumOps<FpType>::HostSumOps;
@@ -50,6 +62,7 @@ public:
     DevSumOps(DevSumOps &&ops) : HostSumOps<FpType>(std::move(ops)) {
         std::swap(handle, ops.handle);
         std::swap(handle_len, ops.handle_len);
+        std::swap(dense, ops.dense);
     }
     template<RealScalar F1>
     explicit DevSumOps(const DevSumOps<F1> &ops) : HostSumOps<FpType>(ops) {}
@@ -74,6 +87,17 @@ public:
         RP_CALL(rp_terms_info(device_terms(len), &info));
         return info;
     }
+
+    // dense forms on len sites (get_matrix, get_eigensystem, matexp;
+    // inc/gpuimpl.h:370-388): cached until the term list changes
+    using MatrixType = DevMatrix<FpType>;
+    using EigenVals = std::vector<FpType>;
+    using EigenSystem = std::tuple<const EigenVals &, const MatrixType &>;
+    const MatrixType &get_matrix(int len) const;
+    EigenSystem get_eigensystem(int len) const;
+    // exp(c H), cached by c (MatexpCache, inc/cpuimpl.h:29-88)
+    const MatrixType &matexp(complex<FpType> c, int len) const;
+    size_t matexp_cache_entries() const;
 };
 
 class GPUImpl {
@@ -289,6 +313,25 @@ public:
                                  size, (double *)h.data(), &nrm, fuse_min, &nz));
         normalized = nz != 0;
         return nrm;
+    }
+
+    // res = mat * vec for a dim x dim device matrix (mat_mul,
+    // inc/gpuimpl.h:1009-1022); res and vec must differ
+    template<RealScalar FpType, typename Mat>
+    static void mat_vec(size_t dim, BufType<FpType> res, const Mat &mat, ConstBufType<FpType> vec) {
+        if (mat.rows() != dim || mat.cols() != dim)
+            ThrowException(InvalidArgument, "matrix dimensions", "must match the state");
+        RP_CALL(rp_dense_gemm(internal::rp_prec<FpType>(), dim, 1, dim, 0, 0, 1.0, 0.0,
+                              mat.data(), dim, vec.ptr, dim, 0.0, 0.0, res.ptr, dim));
+    }
+
+    // v = column idx of a dim x dim device matrix
+    template<RealScalar FpType, typename Mat>
+    static void copy_column(size_t dim, BufType<FpType> v, const Mat &mat, size_t idx) {
+        if (mat.rows() != dim || idx >= mat.cols())
+            ThrowException(InvalidArgument, "column", "out of range");
+        RP_CALL(rp_memcpy(v.ptr, (const complex<FpType> *)mat.data() + idx * dim,
+                          dim * sizeof(complex<FpType>), RP_D2D));
     }
 
     // y += sum_i c[i] V[i] in one sweep

@@ -149,3 +149,37 @@ std::vector<std::complex<double>> npoint_block(NPoint kind, BlockDiag<HamOpTy> &
     for (auto &x : v) x /= v0;
     return v;
 }
+
+// The trace definitions of the same functions (BaseEval::eval with trace
+// set, and evolve_trace; examples/syk.cc:171-185 and 286-298), by exact
+// diagonalization: with expmbH = exp(-beta H_L), m0 = exp(-it H_R),
+// m1 = exp(it H_L) and m2 = op as a dense matrix,
+//     OTOC:  v_k = Tr(m1 m0 m2 m1 m0 m2 expmbH)
+//     Green: v_k = Tr(m1 m0 expmbH)
+// each divided by v_0 (no E0 shift on this path, as in the reference).  The
+// matrix exponentials come from the operators' matexp caches.  For sectors
+// of up to 15 sites.
+template<RealScalar FpType, typename HamOpTy, typename OpTy>
+std::vector<std::complex<double>> npoint_block_trace(NPoint kind, const BlockDiag<HamOpTy> &ham,
+                                                     const OpTy &op, double tmax, int nsteps,
+                                                     double beta)
+{
+    if (nsteps < 0) ThrowException(InvalidArgument, "nsteps", "must not be negative");
+    if (ham.nullLL || ham.nullRR)
+        ThrowException(InvalidArgument, "Hamiltonian", "must have both parity blocks");
+    const auto &expmbH = ham.LL.matexp((FpType)(-beta));
+    const auto &m2 = op.get_matrix();
+    std::vector<std::complex<double>> v;
+    double dt = nsteps ? tmax / nsteps : 0.0;
+    for (int k = 0; k <= nsteps; k++) {
+        FpType t = (FpType)(k * dt);
+        const auto &m0 = ham.RR.matexp(complex<FpType>{0, -t});
+        const auto &m1 = ham.LL.matexp(complex<FpType>{0, t});
+        complex<FpType> tr = kind == NPoint::Otoc ? (m1 * m0 * m2 * m1 * m0 * m2 * expmbH).trace()
+                                                  : (m1 * m0 * expmbH).trace();
+        v.push_back(std::complex<double>(tr));
+    }
+    std::complex<double> v0 = v[0];
+    for (auto &x : v) x /= v0;
+    return v;
+}

@@ -62,6 +62,7 @@ concept ScalarType = RealScalar<T> || ComplexScalar<T>;
 #include "sumops.h"
 #include "expm.h"
 #include "gpuimpl.h"
+#include "dense.h"
 #include "blkops.h"
 
 using DefImpl = GPUImpl;

@@ -1,0 +1,143 @@
+"""Dense matrices and exact diagonalization on the GPU (the reference's
+DevSumOps::get_matrix / get_eigensystem / matexp and ExactDiagonalization,
+inc/gpuimpl.h:1179-1263 and inc/algo.h:28-54) against the oracle's dense
+matrices and numpy/scipy, the Krylov path against ED at sizes the oracle
+cannot reach quickly, and the trace-definition OTOC and Green's function of
+the parity-block SYK driver (examples/syk.cc:171-185, 286-298)."""
+import numpy as np
+import pytest
+import scipy.linalg
+
+from conftest import random_pauli_sum, random_state
+
+pytestmark = pytest.mark.gpu
+
+
+def rounded(c, fp):
+    return c.astype(np.complex64).astype(np.complex128) if fp == 32 else c
+
+
+@pytest.mark.parametrize("fp", [32, 64])
+def test_dense_matrix_matches_oracle(gpu, orc, fp):
+    rng = np.random.default_rng(4)
+    for L, T in [(1, 3), (4, 40), (7, 200)]:
+        bits, coeff = random_pauli_sum(rng, L, T)
+        coeff = rounded(coeff, fp)
+        h = gpu.Hamiltonian(L, bits, coeff, fp=fp)
+        want = orc.dense_matrix(L, bits, coeff)
+        got = h.matrix()
+        tol = 1e-6 * np.abs(coeff).sum() if fp == 32 else 1e-14 * np.abs(coeff).sum()
+        assert np.abs(got - want).max() <= tol, (L, T)
+
+
+@pytest.mark.parametrize("fp", [32, 64])
+def test_eigensystem_and_matexp(gpu, orc, fp):
+    N, L = 14, 6
+    bits, coeff = orc.syk_block_terms(N, 0, seed=8, fp=fp)
+    h = gpu.Hamiltonian(L, bits, coeff, fp=fp)
+    H = orc.dense_matrix(L, bits, coeff)
+    w, V = h.eigensystem()
+    wref = np.linalg.eigvalsh(H)
+    tol = 2e-6 if fp == 32 else 1e-13
+    assert np.all(np.diff(w) >= 0)
+    assert np.abs(w - wref).max() < tol
+    assert np.abs(V.conj().T @ V - np.eye(1 << L)).max() < 10 * tol
+    assert np.abs(H @ V - V * w).max() < 10 * tol
+    for c in (-0.3 - 2.0j, 1.5j, -1.0):
+        E = h.matexp(c)
+        ref = scipy.linalg.expm(c * H)
+        err = np.abs(E - ref).max() / max(1.0, np.abs(ref).max())
+        print("fp%d matexp(%s): rel err %.2e" % (fp, c, err), flush=True)
+        assert err < (2e-5 if fp == 32 else 1e-12)
+    # cached by c: a repeated c adds no entry, a new one does
+    n0 = h.matexp_cache_entries()
+    h.matexp(1.5j)
+    assert h.matexp_cache_entries() == n0
+    h.matexp(0.25j)
+    assert h.matexp_cache_entries() == n0 + 1
+
+
+@pytest.mark.parametrize("fp", [32, 64])
+def test_ed_evolve_and_ground_state(gpu, orc, fp):
+    N, L = 20, 9
+    bits, coeff = orc.syk_block_terms(N, 1, seed=2, fp=fp)
+    h = gpu.Hamiltonian(L, bits, coeff, fp=fp)
+    H = orc.dense_matrix(L, bits, coeff)
+    psi = random_state(np.random.default_rng(6), L)
+    dt = np.complex64 if fp == 32 else np.complex128
+    s = gpu.DeviceState(L, fp=fp)
+    s.load(psi.astype(dt))
+    t, beta = 3.0, 0.4
+    s.evolve_ed(h, t, beta)
+    ref = scipy.linalg.expm((-beta - 1j * t) * H) @ psi
+    err = np.abs(s.store() - ref).max()
+    print("fp%d ED evolve: max err %.2e" % (fp, err), flush=True)
+    assert err < (3e-5 if fp == 32 else 1e-12)
+    e = s.ground_state_ed(h)
+    w = np.linalg.eigvalsh(H)
+    assert e == pytest.approx(w[0], abs=2e-6 if fp == 32 else 1e-12)
+    v = s.store().astype(np.complex128)
+    assert np.linalg.norm(H @ v - e * v) < (1e-4 if fp == 32 else 1e-10)
+
+
+def test_krylov_against_on_gpu_ed(gpu, orc):
+    """The GPU's own ED as the oracle of the Krylov path at 12 sites (4096
+    amplitudes, 1820 terms): FP64 Krylov to 1e-12 and FP32 to 1e-5."""
+    N, L = 26, 12
+    for fp, tol, bound in ((64, 1e-13, 1e-10), (32, 1e-7, 2e-5)):
+        bits, coeff = orc.syk_block_terms(N, 0, sp=4.0, seed=31, fp=fp)
+        h = gpu.Hamiltonian(L, bits, coeff, fp=fp)
+        dt = np.complex64 if fp == 32 else np.complex128
+        psi = random_state(np.random.default_rng(1), L).astype(dt)
+        a, b = gpu.DeviceState(L, fp=fp), gpu.DeviceState(L, fp=fp)
+        a.load(psi)
+        b.load(psi)
+        a.evolve_ed(h, 5.0, 0.2)
+        st = b.evolve(h, 5.0, 0.2, krydim=20, tol=tol)
+        err = np.abs(a.store() - b.store()).max()
+        print("fp%d Krylov vs ED at L=12: %.2e (%d matvecs)" % (fp, err, st.matvecs), flush=True)
+        assert err < bound
+
+
+def dense_trace_npoint(kind, mLL, mRR, mop, tmax, nsteps, beta):
+    E = scipy.linalg.expm(-beta * mLL)
+    out = []
+    for k in range(nsteps + 1):
+        t = k * tmax / nsteps
+        m0 = scipy.linalg.expm(-1j * t * mRR)
+        m1 = scipy.linalg.expm(1j * t * mLL)
+        if kind == "otoc-trace":
+            out.append(np.trace(m1 @ m0 @ mop @ m1 @ m0 @ mop @ E))
+        else:
+            out.append(np.trace(m1 @ m0 @ E))
+    out = np.array(out)
+    return out / out[0]
+
+
+@pytest.mark.parametrize("kind,beta,fp", [("otoc-trace", 0.0, 64), ("otoc-trace", 2.0, 64),
+                                          ("green-trace", 1.0, 64), ("otoc-trace", 2.0, 32)])
+def test_trace_npoint_vs_dense(gpu, orc, kind, beta, fp):
+    N, seed, L = 12, 3, 5
+    tmax, nsteps = 4.0, 8
+    t, v, e0, _ = gpu.syk_block_npoint(N, kind, seed=seed, beta=beta, tmax=tmax, nsteps=nsteps,
+                                       fp=fp)
+    mLL = orc.dense_matrix(L, *orc.syk_block_terms(N, 0, seed=seed, fp=fp))
+    mRR = orc.dense_matrix(L, *orc.syk_block_terms(N, 1, seed=seed, fp=fp))
+    ob, oc = orc.majorana_block(N, N - 2, 0, fp=fp)
+    mop = orc.dense_matrix(L, np.array([ob], np.uint64), np.array([oc]))
+    ref = dense_trace_npoint(kind, mLL, mRR, mop, tmax, nsteps, beta)
+    err = np.abs(v - ref).max()
+    print("%s beta=%g fp%d: max err %.2e" % (kind, beta, fp, err), flush=True)
+    assert v[0] == 1 and e0 == 0
+    assert err < (5e-5 if fp == 32 else 1e-11)
+
+
+def test_trace_and_state_definitions_agree_statistically(gpu):
+    """At beta = 0 the state definition with a random state estimates the
+    trace definition (typicality): 12 sites (N = 26), error ~ 2^-6."""
+    N = 26
+    _, vt, _, _ = gpu.syk_block_npoint(N, "green-trace", seed=7, tmax=3.0, nsteps=3, fp=64)
+    _, vs, _, _ = gpu.syk_block_npoint(N, "green", seed=7, state_seed=5, tmax=3.0, nsteps=3,
+                                       krylov_dim=20, tol=1e-12, fp=64)
+    print("green trace %s\ngreen state %s" % (vt, vs), flush=True)
+    assert np.abs(vt - vs).max() < 0.1
