@@ -198,6 +198,12 @@ def host():
             "rph_state_ground_ed": ([_vp, _vp, _c.POINTER(_dbl)], _c.c_int),
             "rph_ham_dense": ([_vp, _c.c_int, _dbl, _dbl, _vp, _vp], _c.c_int),
             "rph_ham_matexp_entries": ([_vp, _c.POINTER(_i64)], _c.c_int),
+            "rph_ham_to_string": ([_c.c_int, _vp, _vp, _i64, _c.c_char_p, _i64,
+                                   _c.POINTER(_i64)], _c.c_int),
+            "rph_syk_block_dump": ([_c.c_int, _c.c_int, _dbl, _dbl, _c.c_uint32, _c.c_int,
+                                    _c.c_char_p, _i64, _c.POINTER(_i64)], _c.c_int),
+            "rph_format_results": ([_dbl, _vp, _i64, _c.c_char_p, _i64, _c.POINTER(_i64)],
+                                   _c.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(lib, name)
@@ -599,27 +605,6 @@ def syk_block_npoint(N, kind="otoc", sp=0.0, J=1.0, seed=0, regularize=False, ot
     return t, out[0::2] + 1j * out[1::2], e0.value, st
 
 
-def format_result_lines(t, v, digits=6):
-    """Lines 't Re Im', one per time point, as the reference's result files
-    hold them (an ostream's default format: %g with 6 significant digits;
-    digits=17 keeps every bit of a double)."""
-    fmt = "%%.%dg" % digits
-    return ["%s %s %s" % (fmt % float(tk), fmt % float(np.real(vk)), fmt % float(np.imag(vk)))
-            for tk, vk in zip(t, v)]
-
-
-def write_result_file(path, t, v, digits=6):
-    with open(path, "w") as f:
-        for line in format_result_lines(t, v, digits):
-            f.write(line + "\n")
-
-
-def read_result_file(path):
-    """(t, v) from a 't Re Im' result file."""
-    rows = np.loadtxt(path, ndmin=2)
-    return rows[:, 0], rows[:, 1] + 1j * rows[:, 2]
-
-
 # ---------------------------------------------------------------------------
 # Krylov schedules and the proposer network
 # ---------------------------------------------------------------------------
@@ -815,3 +800,7 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+from .formats import (format_result_lines, ham_to_string, parse_ham_string,  # noqa: E402
+                      read_result_file, service_request, syk_block_dump, write_result_file)

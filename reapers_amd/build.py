@@ -4,6 +4,8 @@
                        include/reapers_hip.h (hipcc, one object per .hip file)
 - libreapers_host.so : the header-only C++ API instantiated for Python callers
                        (g++, links libreapers_hip.so)
+- reapers_service    : the JSON request/response program (g++, links
+                       libreapers_hip.so)
 
 Run ``python -m reapers_amd.build`` or call :func:`build_all`.
 """
@@ -92,6 +94,21 @@ def build_host(force=False):
     return lib
 
 
+def build_service(force=False):
+    """reapers_service: the JSON request/response program (csrc/service.cc)."""
+    deps = (_headers(os.path.join(HERE, "include"))
+            + [os.path.join(ROOT, "include", "reapers_hip.h"),
+               os.path.join(HERE, "libreapers_hip.so")])
+    src = os.path.join(CSRC, "service.cc")
+    exe = os.path.join(HERE, "reapers_service")
+    if force or _stale(exe, [src] + deps):
+        tmp = exe + ".tmp"
+        _run([CXX] + HOST_FLAGS + ["-o", tmp, src, "-L" + HERE, "-lreapers_hip",
+                                   "-Wl,-rpath,$ORIGIN"])
+        os.replace(tmp, exe)
+    return exe
+
+
 def build_oracle(force=False):
     odir = os.path.join(ROOT, "oracle")
     if force:
@@ -104,6 +121,7 @@ def build_oracle(force=False):
 def build_all(force=False):
     build_hip(force)
     build_host(force)
+    build_service(force)
     build_oracle(force)
 
 

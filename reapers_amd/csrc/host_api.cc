@@ -1075,3 +1075,67 @@ int rph_ham_matexp_entries(void *hp, int64_t *n)
 }
 
 }  // extern "C"
+
+// text forms ----------------------------------------------------------------
+
+namespace {
+int put_string(const std::string &s, char *buf, int64_t cap, int64_t *len)
+{
+    *len = (int64_t)s.size();
+    if (buf && cap > 0) {
+        int64_t n = std::min<int64_t>(cap - 1, (int64_t)s.size());
+        std::memcpy(buf, s.data(), n);
+        buf[n] = 0;
+    }
+    return 0;
+}
+}  // namespace
+
+extern "C" {
+
+// SumOps' operator<< of a term list; *len receives the full length (the
+// text is cut to cap - 1 characters)
+int rph_ham_to_string(int fp, const uint64_t *bits, const double *coeff, int64_t n, char *buf,
+                      int64_t cap, int64_t *len)
+{
+    return guarded([&] {
+        std::stringstream ss;
+        if (fp == 32) ss << HostSumOps<float>::from_terms(bits, coeff, n);
+        else ss << HostSumOps<double>::from_terms(bits, coeff, n);
+        put_string(ss.str(), buf, cap, len);
+    });
+}
+
+// the SYK driver's Hamiltonian dump of SYKBlockForm(N, sp, J).gen_ham(
+// std::mt19937(seed), regularize)
+int rph_syk_block_dump(int fp, int N, double sp, double J, uint32_t seed, int regularize,
+                       char *buf, int64_t cap, int64_t *len)
+{
+    return guarded([&] {
+        std::stringstream ss;
+        std::mt19937 rg(seed);
+        if (fp == 32) {
+            SYKBlockForm<float, GPUImpl> syk(N, (float)sp, (float)J);
+            dump_block_ham(ss, syk.gen_ham(rg, regularize != 0));
+        } else {
+            SYKBlockForm<double, GPUImpl> syk(N, sp, J);
+            dump_block_ham(ss, syk.gen_ham(rg, regularize != 0));
+        }
+        put_string(ss.str(), buf, cap, len);
+    });
+}
+
+// write_results of n values (interleaved re, im) at spacing dt
+int rph_format_results(double dt, const double *v, int64_t n, char *buf, int64_t cap,
+                       int64_t *len)
+{
+    return guarded([&] {
+        std::vector<std::complex<double>> c(n);
+        for (int64_t k = 0; k < n; k++) c[k] = {v[2 * k], v[2 * k + 1]};
+        std::stringstream ss;
+        write_results(ss, dt, c);
+        put_string(ss.str(), buf, cap, len);
+    });
+}
+
+}  // extern "C"

@@ -123,6 +123,7 @@ public:
     }
     using SumOpsTy::matexp;
     using SumOpsTy::get_matrix;
+    using SumOpsTy::get_eigensystem;
 };
 
 template<RealScalar FpType = DefFpType, typename Impl = DefImpl>

@@ -183,3 +183,22 @@ std::vector<std::complex<double>> npoint_block_trace(NPoint kind, const BlockDia
     for (auto &x : v) x /= v0;
     return v;
 }
+
+// The reference's text outputs of the SYK driver: the Hamiltonian dump
+// ("H_LL = <LL>" and "H_RR = <RR>", one line each, in SumOps' operator<<
+// form; examples/syk.cc:429-438) and the result file (one "t Re Im" line
+// per time point t = k dt, each number in an ostream's default format;
+// examples/syk.cc:230-235).
+template<typename HamOpTy>
+void dump_block_ham(std::ostream &os, const BlockDiag<HamOpTy> &ham)
+{
+    os << "H_LL = " << ham.LL << std::endl;
+    os << "H_RR = " << ham.RR << std::endl;
+}
+
+template<typename C>
+void write_results(std::ostream &os, double dt, const std::vector<C> &v)
+{
+    for (size_t k = 0; k < v.size(); k++)
+        os << dt * k << " " << v[k].real() << " " << v[k].imag() << std::endl;
+}
