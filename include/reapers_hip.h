@@ -196,7 +196,12 @@ int rp_dense_build(int prec, int len, const uint64_t *bits, const double *coeff,
 /* Hermitian eigensystem (cusolverDnXsyevd in get_eigensystem,
  * inc/gpuimpl.h:1179-1233): the lower triangle of mat is read, mat is
  * overwritten by the column eigenvectors and w (device, n reals of the
- * precision) receives the eigenvalues in ascending order */
+ * precision) receives the eigenvalues in ascending order.  rocSOLVER's
+ * divide-and-conquer heevd runs first; if its eigenvectors are not finite
+ * (it fails so on matrices made of decoupled blocks) it runs again on
+ * U^H A U, U a Walsh-Hadamard matrix (n a power of two) times random phases,
+ * and the eigenvectors are rotated back; heev (QR iteration) is the last
+ * resort. */
 int rp_dense_eigh(int prec, uint64_t n, void *mat, void *w);
 /* exp(c H) = V diag(exp(c w)) V^H from H's eigensystem (_matexp,
  * inc/gpuimpl.h:1235-1252); work is n*n scratch elements; res, work and V

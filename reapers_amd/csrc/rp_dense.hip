@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <numeric>
 #include <vector>
 
@@ -91,6 +92,31 @@ k_eye(typename Cx2<R>::T *__restrict__ mat, uint64_t n)
         v.x = (i % (n + 1) == 0) ? (R)1 : (R)0;
         v.y = (R)0;
         mat[i] = v;
+    }
+}
+
+// U[r + c n] = (-1)^popcount(r & c) / sqrt(n) * exp(i theta_c): a
+// Walsh-Hadamard matrix times random phases (theta_c from a splitmix64 hash
+// of c and the seed), unitary for n a power of two
+template <typename R>
+__global__ void __launch_bounds__(kDenseThreads)
+k_mixing_unitary(typename Cx2<R>::T *__restrict__ U, uint64_t n, uint64_t seed)
+{
+    uint64_t total = n * n;
+    double inv = 1.0 / sqrt((double)n);
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t r = i % n, c = i / n;
+        uint64_t z = (c + 1) * 0x9e3779b97f4a7c15ull + seed;
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        z ^= z >> 31;
+        double theta = 6.283185307179586 * (double)(z >> 11) * (1.0 / 9007199254740992.0);
+        double sg = (__popcll(r & c) & 1) ? -inv : inv;
+        typename Cx2<R>::T v;
+        v.x = (R)(sg * cos(theta));
+        v.y = (R)(sg * sin(theta));
+        U[i] = v;
     }
 }
 
@@ -231,6 +257,41 @@ int rp_dense_build(int prec, int len, const uint64_t *bits, const double *coeff,
     return rc;
 }
 
+// heevd (divide and conquer) first.  rocSOLVER's heevd returns non-finite
+// eigenvectors for matrices that split into decoupled blocks, such as a
+// parity-conserving Hamiltonian written in the computational basis (its
+// tridiagonal form splits).  Then the saved matrix is rotated to
+// A' = U^H A U with a Walsh-Hadamard matrix times random phases, whose
+// tridiagonal form does not split, heevd solves A' and the eigenvectors are
+// U V'.  QR iteration (heev) is the last resort.
+static int eigh_call(int prec, rocblas_handle h, bool dc, uint64_t n, void *mat, void *w,
+                     void *e, rocblas_int *dinfo, rocblas_status *s)
+{
+    if (prec == RP_C64)
+        *s = dc ? rocsolver_cheevd(h, rocblas_evect_original, rocblas_fill_lower, (rocblas_int)n,
+                                   (rocblas_float_complex *)mat, (rocblas_int)n, (float *)w,
+                                   (float *)e, dinfo)
+                : rocsolver_cheev(h, rocblas_evect_original, rocblas_fill_lower, (rocblas_int)n,
+                                  (rocblas_float_complex *)mat, (rocblas_int)n, (float *)w,
+                                  (float *)e, dinfo);
+    else
+        *s = dc ? rocsolver_zheevd(h, rocblas_evect_original, rocblas_fill_lower, (rocblas_int)n,
+                                   (rocblas_double_complex *)mat, (rocblas_int)n, (double *)w,
+                                   (double *)e, dinfo)
+                : rocsolver_zheev(h, rocblas_evect_original, rocblas_fill_lower, (rocblas_int)n,
+                                  (rocblas_double_complex *)mat, (rocblas_int)n, (double *)w,
+                                  (double *)e, dinfo);
+    if (*s != rocblas_status_success) return blas_error(dc ? "rocsolver heevd" : "rocsolver heev", *s);
+    rocblas_int info = 0;
+    RP_CHECK_HIP(hipMemcpyAsync(ctx().pinned, dinfo, sizeof(info), hipMemcpyDeviceToHost,
+                                ctx().stream));
+    RP_CHECK_HIP(hipStreamSynchronize(ctx().stream));
+    std::memcpy(&info, ctx().pinned, sizeof(info));
+    if (info != 0)
+        return set_error(RP_ERR_RUNTIME, "rocsolver %s: info = %d", dc ? "heevd" : "heev", (int)info);
+    return RP_OK;
+}
+
 int rp_dense_eigh(int prec, uint64_t n, void *mat, void *w)
 {
     int st = check_init();
@@ -242,29 +303,76 @@ int rp_dense_eigh(int prec, uint64_t n, void *mat, void *w)
     if (!h) return set_error(RP_ERR_RUNTIME, "rocblas_create_handle failed");
     Context &c = ctx();
     size_t rbytes = prec == RP_C64 ? sizeof(float) : sizeof(double);
-    void *work = nullptr;
+    size_t mbytes = n * n * elem_bytes(prec);
+    void *work = nullptr, *saved = nullptr, *U = nullptr, *tmp = nullptr;
     RP_CHECK_HIP(hipMalloc(&work, n * rbytes + sizeof(rocblas_int)));
-    rocblas_int *dinfo = (rocblas_int *)((char *)work + n * rbytes);
-    rocblas_status s;
-    if (prec == RP_C64)
-        s = rocsolver_cheevd(h, rocblas_evect_original, rocblas_fill_lower, (rocblas_int)n,
-                             (rocblas_float_complex *)mat, (rocblas_int)n, (float *)w,
-                             (float *)work, dinfo);
-    else
-        s = rocsolver_zheevd(h, rocblas_evect_original, rocblas_fill_lower, (rocblas_int)n,
-                             (rocblas_double_complex *)mat, (rocblas_int)n, (double *)w,
-                             (double *)work, dinfo);
-    rocblas_int info = 0;
-    hipError_t e = hipSuccess;
-    if (s == rocblas_status_success) {
-        e = hipMemcpyAsync(&info, dinfo, sizeof(info), hipMemcpyDeviceToHost, c.stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+    if (hipMalloc(&saved, mbytes) != hipSuccess) {
+        (void)hipFree(work);
+        return set_error(RP_ERR_OUT_OF_MEMORY, "rp_dense_eigh: no memory for the matrix copy");
     }
+    rocblas_int *dinfo = (rocblas_int *)((char *)work + n * rbytes);
+    int rc = RP_OK;
+    rocblas_status s;
+    auto finite = [&](bool &ok) {
+        double nrm = 0;
+        int r = rp_vec_norm(prec, mat, n * n, &nrm);
+        ok = std::isfinite(nrm);
+        return r;
+    };
+    auto restore = [&]() -> int {
+        RP_CHECK_HIP(hipMemcpyAsync(mat, saved, mbytes, hipMemcpyDeviceToDevice, c.stream));
+        return RP_OK;
+    };
+    do {
+        RP_CHECK_HIP(hipMemcpyAsync(saved, mat, mbytes, hipMemcpyDeviceToDevice, c.stream));
+        rc = eigh_call(prec, h, true, n, mat, w, work, dinfo, &s);
+        if (rc) break;
+        bool ok = false;
+        if ((rc = finite(ok)) || ok) break;
+        bool pow2 = (n & (n - 1)) == 0;
+        // rotated (n a power of two): tmp = A U, mat = U^H tmp, heevd, then
+        // V = U V' into mat
+        if (pow2 && (hipMalloc(&U, mbytes) != hipSuccess || hipMalloc(&tmp, mbytes) != hipSuccess)) {
+            rc = set_error(RP_ERR_OUT_OF_MEMORY, "rp_dense_eigh: no memory for the rotation");
+            break;
+        }
+        if (!pow2) {
+            if ((rc = restore())) break;
+            rc = eigh_call(prec, h, false, n, mat, w, work, dinfo, &s);
+            if (!rc && !(rc = finite(ok)) && !ok)
+                rc = set_error(RP_ERR_RUNTIME, "rp_dense_eigh: eigenvectors are not finite");
+            break;
+        }
+        if (prec == RP_C64)
+            hipLaunchKernelGGL(k_mixing_unitary<float>, dim3(grid_of(n * n)), dim3(kDenseThreads), 0,
+                               c.stream, (float2 *)U, n, (uint64_t)0x5eedf00dull);
+        else
+            hipLaunchKernelGGL(k_mixing_unitary<double>, dim3(grid_of(n * n)), dim3(kDenseThreads), 0,
+                               c.stream, (double2 *)U, n, (uint64_t)0x5eedf00dull);
+        RP_CHECK_LAUNCH();
+        if ((rc = rp_dense_gemm(prec, n, n, n, 0, 0, 1, 0, saved, n, U, n, 0, 0, tmp, n))) break;
+        if ((rc = rp_dense_gemm(prec, n, n, n, 1, 0, 1, 0, U, n, tmp, n, 0, 0, mat, n))) break;
+        rc = eigh_call(prec, h, true, n, mat, w, work, dinfo, &s);
+        if (rc) break;
+        if ((rc = finite(ok))) break;
+        if (ok) {
+            RP_CHECK_HIP(hipMemcpyAsync(tmp, mat, mbytes, hipMemcpyDeviceToDevice, c.stream));
+            rc = rp_dense_gemm(prec, n, n, n, 0, 0, 1, 0, U, n, tmp, n, 0, 0, mat, n);
+            break;
+        }
+        // last resort: QR iteration on the original matrix
+        if ((rc = restore())) break;
+        rc = eigh_call(prec, h, false, n, mat, w, work, dinfo, &s);
+        if (rc) break;
+        if ((rc = finite(ok))) break;
+        if (!ok) rc = set_error(RP_ERR_RUNTIME, "rp_dense_eigh: eigenvectors are not finite");
+    } while (0);
+    (void)hipStreamSynchronize(c.stream);
+    if (tmp) (void)hipFree(tmp);
+    if (U) (void)hipFree(U);
+    (void)hipFree(saved);
     (void)hipFree(work);
-    if (s != rocblas_status_success) return blas_error("rocsolver heevd", s);
-    if (e != hipSuccess) return set_error(RP_ERR_RUNTIME, "rp_dense_eigh: %s", hipGetErrorString(e));
-    if (info != 0) return set_error(RP_ERR_RUNTIME, "rocsolver heevd: info = %d", (int)info);
-    return RP_OK;
+    return rc;
 }
 
 int rp_dense_expm_eig(int prec, uint64_t n, const void *V, const void *w, double c_re,

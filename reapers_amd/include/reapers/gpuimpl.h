@@ -43,12 +43,16 @@ class DevSumOps : public HostSumOps<FpType> {
     mutable int handle_len = -1;
     mutable std::unique_ptr<internal::DenseCache<FpType>> dense;
 
-    void release() const override {
+    void free_handle() const {
         if (handle) {
             rp_terms_free(handle);
             handle = nullptr;
             handle_len = -1;
         }
+    }
+
+    void release() const override {
+        free_handle();
         dense.reset();
     }
 
@@ -349,7 +353,7 @@ public:
 template<RealScalar FpType>
 inline rp_terms_t DevSumOps<FpType>::device_terms(int len) const {
     if (handle && handle_len == len) return handle;
-    release();
+    free_handle();
     GPUImpl::ensure_context();
     std::vector<u64> bits;
     std::vector<double> coeff;

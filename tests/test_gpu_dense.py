@@ -57,6 +57,24 @@ def test_eigensystem_and_matexp(gpu, orc, fp):
     assert h.matexp_cache_entries() == n0 + 1
 
 
+@pytest.mark.parametrize("N,sp", [(20, 0.0), (24, 0.0), (24, 8.0)])
+def test_eigensystem_of_parity_conserving_matrix(gpu, orc, N, sp):
+    """The non-block SYK Hamiltonian conserves parity, so in the computational
+    basis its matrix splits into decoupled blocks (and for N = 20 every level
+    is doubly degenerate); the eigenvectors must come back finite and solve
+    the eigenproblem, and the exponential must match scipy's."""
+    L = N // 2
+    bits, coeff = orc.syk_terms(N, sp, seed=2000, fp=64)
+    h = gpu.Hamiltonian(L, bits, coeff, fp=64)
+    H = orc.dense_matrix(L, bits, coeff)
+    w, V = h.eigensystem()
+    assert np.all(np.isfinite(V))
+    assert np.abs(w - np.linalg.eigvalsh(H)).max() < 1e-12
+    assert np.abs(H @ V - V * w).max() < 1e-11
+    E = h.matexp(-0.5j)
+    assert np.abs(E - scipy.linalg.expm(-0.5j * H)).max() < 1e-11
+
+
 @pytest.mark.parametrize("fp", [32, 64])
 def test_ed_evolve_and_ground_state(gpu, orc, fp):
     N, L = 20, 9
