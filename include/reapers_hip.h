@@ -135,20 +135,26 @@ int rp_terms_free(rp_terms_t h);
 typedef struct {
     int      prec;          /* RP_C64 / RP_C128 */
     int      len;           /* spin chain length */
-    int      tile_bits;     /* b: log2 of the amplitude tile one workgroup owns */
+    int      tile_bits;     /* b: log2 of the amplitude tile one workgroup owns
+                               (0: one thread per amplitude, no tiles) */
     uint64_t nterms;        /* terms after dropping zero coefficients */
     uint64_t nflips;        /* distinct flip masks x (G_0) */
     uint64_t ngroups;       /* distinct x >> b: input tile passes (G_b) */
     uint64_t bytes_per_apply; /* algorithmic HBM bytes (G_b + 1) * 2^len * S */
     int      threads;       /* workgroup size of the tiled kernel */
-    int      depth;         /* LDS tiles in its pipeline (input tiles in flight + 1) */
+    int      depth;         /* LDS tiles in its pipeline (input tiles in flight + 1);
+                               for tile_bits 0 the number of term slices */
 } rp_terms_info_t;
 int rp_terms_info(rp_terms_t h, rp_terms_info_t *info);
 /* Rebuild the layout for a given tiling of the tiled kernel (no counterpart
  * in the reference): tile bits / threads / pipeline depth must be one of the
  * instantiated combinations -- (8,128,2), (10,128,2), (12,512,2) for both
  * precisions; (12,512,3), (12,512,4), (13,1024,2) for RP_C64; (11,256,4) for
- * RP_C128.  The autotuner (RP_TERMS_AUTOTUNE) chooses among the same set. */
+ * RP_C128 -- or tile bits 0 with 256 threads: the one-thread-per-amplitude
+ * kernel, its depth the number of term slices (1, 8, 32, 128) that
+ * separate workgroups sum before a second pass adds them in slice order.
+ * The autotuner (RP_TERMS_AUTOTUNE) chooses among the same set, the sliced
+ * kernel only up to len 16. */
 int rp_terms_set_tiling(rp_terms_t h, int tile_bits, int threads, int depth);
 
 /* GPUImpl::apply_ops + krnl_apply_ops (src/krnl.cu:220-257, 80-94):

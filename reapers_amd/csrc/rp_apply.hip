@@ -51,11 +51,14 @@ struct rp_terms {
     int len = 0;
     int tile_bits = 0;       // 0: small kernel (one thread per amplitude)
     int threads = 0;         // workgroup size of the tiled kernel
-    int depth = 2;           // LDS tiles in the tiled kernel's pipeline
+    int depth = 2;           // LDS tiles in the tiled kernel's pipeline; for the
+                             // small kernel the number of term slices (1: none)
     int acc64 = 1;           // accumulate in FP64
     int group_acc = 0;       // FP32 within a group, FP64 across groups (complex<float>)
     uint64_t nterms = 0, nflips = 0, ngroups = 0, nsubs = 0;
     void *d_groups = nullptr, *d_subs = nullptr, *d_terms = nullptr, *d_flat = nullptr;
+    void *d_split = nullptr;          // per-slice partial sums of the split small kernel
+    size_t split_bytes = 0;
     std::vector<uint64_t> raw_bits;   // the uploaded term list, for re-tiling
     std::vector<double> raw_coeff;
 };
@@ -90,6 +93,62 @@ k_apply_small(const typename CxA<R>::T *__restrict__ in, typename CxA<R>::T *__r
     o.x = (R)ar;
     o.y = (R)ai;
     out[m] = o;
+}
+
+// The small kernel with the terms split into gridDim.y slices: slice s sums
+// its contiguous range of the x-sorted terms for one amplitude per thread and
+// writes the partial sum; k_split_reduce adds the slices in slice order.  For
+// chains of up to 16 sites, where 2^len / 256 workgroups alone cannot fill
+// the GPU.
+template <typename R, typename A>
+__global__ void __launch_bounds__(kSmallThreads)
+k_apply_split(const typename CxA<R>::T *__restrict__ in, typename CxA<A>::T *__restrict__ part,
+              uint32_t dim, const FlatTerm<A> *__restrict__ terms, int nterms, int per)
+{
+    uint32_t m = blockIdx.x * kSmallThreads + threadIdx.x;
+    if (m >= dim) return;
+    const int t0 = blockIdx.y * per;
+    const int t1 = min(nterms, t0 + per);
+    A ar = 0, ai = 0;
+    for (int t = t0; t < t1; t++) {
+        const FlatTerm<A> f = terms[t];
+        const auto v = in[m ^ f.x];
+        const A vr = v.x, vi = v.y;
+        const bool neg = parity32(m & f.z);
+        const A cr = neg ? -f.cr : f.cr;
+        const A ci = neg ? -f.ci : f.ci;
+        ar += cr * vr - ci * vi;
+        ai += cr * vi + ci * vr;
+    }
+    typename CxA<A>::T o;
+    o.x = ar;
+    o.y = ai;
+    part[(uint64_t)blockIdx.y * dim + m] = o;
+}
+
+template <typename R, typename A>
+__global__ void __launch_bounds__(kSmallThreads)
+k_split_reduce(const typename CxA<A>::T *__restrict__ part, typename CxA<R>::T *__restrict__ out,
+               uint32_t dim, int slices, int accumulate)
+{
+    for (uint32_t m = blockIdx.x * kSmallThreads + threadIdx.x; m < dim;
+         m += gridDim.x * kSmallThreads) {
+        A ar = 0, ai = 0;
+        for (int sl = 0; sl < slices; sl++) {
+            const auto p = part[(uint64_t)sl * dim + m];
+            ar += p.x;
+            ai += p.y;
+        }
+        if (accumulate) {
+            const auto o = out[m];
+            ar += (A)o.x;
+            ai += (A)o.y;
+        }
+        typename CxA<R>::T o;
+        o.x = (R)ar;
+        o.y = (R)ai;
+        out[m] = o;
+    }
 }
 
 template <typename R, int TB, int NT, int D = 2>
@@ -404,7 +463,11 @@ constexpr int tiling_key(int tb, int nt, int d) { return tb * 100000 + nt * 10 +
 // the tilings the tiled kernel is instantiated for: {tile bits, threads,
 // pipeline depth, element bytes (0: both precisions)}
 struct TilingDesc { int tb, nt, depth, esz; };
+// Tile bits 0 is the small kernel, its depth the number of term slices.
+constexpr int kSplitMaxLen = 16;
 constexpr TilingDesc kTilings[] = {
+    {0, kSmallThreads, 1, 0}, {0, kSmallThreads, 8, 0}, {0, kSmallThreads, 32, 0},
+    {0, kSmallThreads, 128, 0},
     {8, 128, 2, 0}, {10, 128, 2, 0}, {11, 256, 4, 16}, {12, 512, 2, 0},
     {12, 512, 3, 8}, {12, 512, 4, 8}, {13, 1024, 2, 8}};
 
@@ -417,20 +480,35 @@ static bool tiling_instantiated(int prec, int tb, int nt, int d)
     return false;
 }
 
-// default tile bits b and workgroup size (RP_TERMS_AUTOTUNE measures the
-// alternatives): 8 amplitudes per thread, smaller tiles and workgroups for
-// short chains so that there are enough workgroups
-static void choose_tiling(int prec, int len, int *tb, int *nt)
+// default tiling (RP_TERMS_AUTOTUNE measures the alternatives).  Up to 16
+// sites the small kernel, its terms split into slices when 2^len / 256
+// workgroups are too few for the term count: the largest power-of-two slice
+// count up to 128 that keeps at most 2048 workgroups and 32 terms per slice.
+// Beyond, the tiled kernel: 8 amplitudes per thread, smaller tiles and
+// workgroups for shorter chains so that there are enough workgroups.
+static void choose_tiling(int prec, int len, uint64_t nterms, int *tb, int *nt, int *depth)
 {
+    (void)prec;
+    *depth = 2;
+    if (len <= kSplitMaxLen) {
+        uint64_t blocks = std::max<uint64_t>(1, (1ull << len) / kSmallThreads);
+        int sl = 1;
+        while (sl < 128 && blocks * (uint64_t)(2 * sl) <= 2048 && nterms >= (uint64_t)(64 * sl))
+            sl *= 2;
+        if (sl > 1 || len < 10) {
+            *tb = 0;
+            *nt = kSmallThreads;
+            *depth = sl;
+            return;
+        }
+    }
     // 2^12-amplitude tiles with 512 threads: for complex<float> sparse SYK at
     // 24 qubits faster than 2^13 tiles (more groups, but two workgroups per
     // CU instead of one); for complex<double> the largest tile in 128 KiB
-    (void)prec;
     if (len >= 18) { *tb = 12; *nt = 512; return; }
     if (len >= 14) { *tb = 10; *nt = 128; return; }
-    if (len >= 10) { *tb = 8; *nt = 128; return; }
-    *tb = 0;
-    *nt = kSmallThreads;
+    *tb = 8;
+    *nt = 128;
 }
 
 template <typename R, typename A>
@@ -462,6 +540,21 @@ static int upload_impl(rp_terms *h, const uint64_t *bits, const double *coeff, u
         return RP_OK;
     };
     if ((st = up(&h->d_flat, flat.data(), flat.size() * sizeof(FlatTerm<A>)))) return st;
+    if (tb == 0 && h->depth > 1) {
+        size_t need = (size_t)h->depth * (1ull << h->len) * sizeof(typename CxA<A>::T);
+        if (h->split_bytes < need) {
+            if (h->d_split) (void)hipFree(h->d_split);
+            h->d_split = nullptr;
+            h->split_bytes = 0;
+            if (hipMalloc(&h->d_split, need) != hipSuccess) {
+                (void)hipGetLastError();
+                h->d_split = nullptr;
+                return set_error(RP_ERR_OUT_OF_MEMORY, "Failed to allocate %zu bytes on GPU device.",
+                                 need);
+            }
+            h->split_bytes = need;
+        }
+    }
     if (tb) {
         if ((st = up(&h->d_groups, groups.data(), groups.size() * sizeof(GroupRuns)))) return st;
         if ((st = up(&h->d_subs, subs.data(), subs.size() * sizeof(SubRec<A>)))) return st;
@@ -478,6 +571,7 @@ static void free_handle(rp_terms *h)
     if (h->d_subs) (void)hipFree(h->d_subs);
     if (h->d_terms) (void)hipFree(h->d_terms);
     if (h->d_flat) (void)hipFree(h->d_flat);
+    if (h->d_split) (void)hipFree(h->d_split);
     delete h;
 }
 
@@ -513,6 +607,17 @@ static int apply_impl(rp_terms *h, void *res, const void *vec, int accumulate)
     }
     if (h->tile_bits == 0) {
         int grid = (int)((dim + kSmallThreads - 1) / kSmallThreads);
+        if (h->depth > 1) {
+            using P = typename CxA<A>::T;
+            const int slices = h->depth;
+            const int per = (int)((h->nterms + slices - 1) / slices);
+            RP_LAUNCH(RP_KERNEL_APPLY, (k_apply_split<R, A>), dim3(grid, slices),
+                      dim3(kSmallThreads), 0, (const T *)vec, (P *)h->d_split, (uint32_t)dim,
+                      (const FlatTerm<A> *)h->d_flat, (int)h->nterms, per);
+            RP_LAUNCH(RP_KERNEL_APPLY, (k_split_reduce<R, A>), dim3(grid), dim3(kSmallThreads), 0,
+                      (const P *)h->d_split, (T *)res, (uint32_t)dim, slices, accumulate);
+            return RP_OK;
+        }
         RP_LAUNCH(RP_KERNEL_APPLY, (k_apply_small<R, A>), dim3(grid), dim3(kSmallThreads), 0,
                   (const T *)vec, (T *)res, (uint32_t)dim, (const FlatTerm<A> *)h->d_flat,
                   (int)h->nterms, accumulate);
@@ -596,6 +701,7 @@ static int autotune(rp_terms *h)
     for (const TilingDesc &cd : kTilings) {
         if (st) break;
         if (cd.tb > h->len || !tiling_instantiated(h->prec, cd.tb, cd.nt, cd.depth)) continue;
+        if (cd.tb == 0 && (h->len > kSplitMaxLen || h->nterms < (uint64_t)(2 * cd.depth))) continue;
         if ((st = retile(h, cd.tb, cd.nt, cd.depth))) break;
         if ((st = apply_dispatch(h, y, x, 0))) break;
         if (hipEventRecord(e0, c.stream) != hipSuccess) { st = RP_ERR_RUNTIME; break; }
@@ -640,11 +746,11 @@ int rp_terms_upload_ex(rp_terms_t *out, int prec, int len, const uint64_t *bits,
     h->len = len;
     h->acc64 = (prec == RP_C128 || !(flags & RP_TERMS_ACC_NATIVE)) ? 1 : 0;
     h->group_acc = (prec == RP_C64 && (flags & RP_TERMS_ACC_GROUP)) ? 1 : 0;
-    choose_tiling(prec, len, &h->tile_bits, &h->threads);
+    choose_tiling(prec, len, nterms, &h->tile_bits, &h->threads, &h->depth);
     h->raw_bits.assign(bits, bits + nterms);
     h->raw_coeff.assign(coeff, coeff + 2 * nterms);
     st = upload_any(h, bits, coeff, nterms);
-    if (!st && (flags & RP_TERMS_AUTOTUNE) && h->tile_bits && h->nterms && len <= 31)
+    if (!st && (flags & RP_TERMS_AUTOTUNE) && h->nterms && len <= 31)
         st = autotune(h);
     if (st) {
         free_handle(h);

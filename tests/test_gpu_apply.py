@@ -106,21 +106,25 @@ def test_autotuned_upload_matches_oracle(gpu, orc, fp, L):
     got = s.store().astype(np.complex128)
     print("autotuned L=%d fp%d: tile bits %d" % (L, fp, info.tile_bits), flush=True)
     assert (info.tile_bits, info.threads, info.depth) in [
+        (0, 256, 1), (0, 256, 8), (0, 256, 32), (0, 256, 128),
         (8, 128, 2), (10, 128, 2), (11, 256, 4), (12, 512, 2), (12, 512, 3), (12, 512, 4),
         (13, 1024, 2)]
     assert np.abs(got - ref).max() < (1e-5 if fp == 32 else 1e-12) * np.abs(ref).max()
 
 
-TILINGS = {32: [(8, 128, 2), (10, 128, 2), (12, 512, 2), (12, 512, 3), (12, 512, 4),
-                (13, 1024, 2)],
-           64: [(8, 128, 2), (10, 128, 2), (11, 256, 4), (12, 512, 2)]}
+SMALL = [(0, 256, 1), (0, 256, 8), (0, 256, 32), (0, 256, 128)]   # terms split in slices
+TILINGS = {32: SMALL + [(8, 128, 2), (10, 128, 2), (12, 512, 2), (12, 512, 3), (12, 512, 4),
+                        (13, 1024, 2)],
+           64: SMALL + [(8, 128, 2), (10, 128, 2), (11, 256, 4), (12, 512, 2)]}
 
 
 @pytest.mark.parametrize("fp", [32, 64])
 def test_every_tiling_matches_oracle(gpu, orc, fp):
     """Every instantiated (tile bits, threads, pipeline depth) of the tiled
-    kernel, set through rp_terms_set_tiling, on SYK (real/imaginary subs,
-    many groups) and a general sum (complex and multi-term subs)."""
+    kernel and every term-slice count of the small kernel (tile bits 0), set
+    through rp_terms_set_tiling, on SYK (real/imaginary subs, many groups)
+    and a general sum (complex and multi-term subs), overwriting and
+    accumulating."""
     import ctypes
     from gpu_util import DeviceBuffer, Terms, prec_of
     rng = np.random.default_rng(fp)
@@ -143,6 +147,10 @@ def test_every_tiling_matches_oracle(gpu, orc, fp):
             err = np.abs(got - ref).max() / np.abs(ref).max()
             print("L=%d fp%d tiling %s: %.2e" % (L, fp, (tb, nt, d), err), flush=True)
             assert err < (1e-6 if fp == 32 else 1e-13)
+            # accumulate: res += H vec on top of the previous result
+            gpu.check(gpu.hip().rp_apply_ops(t.h, vout.ptr, vin.ptr, 1))
+            got2 = vout.to_array(dt, len(psi)).astype(np.complex128)
+            assert np.abs(got2 - 2 * ref).max() / np.abs(ref).max() < (2e-6 if fp == 32 else 1e-13)
         # a tiling that is not instantiated is an error, not a launch
         assert gpu.hip().rp_terms_set_tiling(t.h, 9, 128, 2) != 0
         vin.free(); vout.free(); t.free()
@@ -177,3 +185,22 @@ def test_group_accumulation_matches_oracle(gpu, orc, tiling):
     # a few FP32 ulps of the largest amplitude (FP32 accumulation: ~1e-6)
     assert err < 4e-7
     gpu.check(gpu.hip().rp_terms_free(h))
+
+
+@pytest.mark.parametrize("fp", [32, 64])
+def test_default_split_tiling_dense_syk24(gpu, orc, fp):
+    """Dense SYK N=24 (10626 strings on 4096 amplitudes) gets the term-split
+    small kernel by default; its result matches the oracle."""
+    L = 12
+    bits, coeff = orc.syk_terms(24, 0.0, seed=5, fp=fp)
+    psi = random_state(np.random.default_rng(9), L)
+    ham = gpu.Hamiltonian(L, bits, coeff, fp=fp)
+    info = ham.info()
+    assert (info.tile_bits, info.threads) == (0, 256) and info.depth > 1
+    s = gpu.DeviceState(L, fp=fp)
+    s.load(psi.astype(np.complex64 if fp == 32 else np.complex128))
+    s.apply(ham)
+    ref = orc.apply(L, bits, coeff, psi, fp=64)
+    err = np.abs(s.store().astype(np.complex128) - ref).max() / np.abs(ref).max()
+    print("dense SYK N=24 fp%d, %d slices: %.2e" % (fp, info.depth, err), flush=True)
+    assert err < (1e-6 if fp == 32 else 1e-13)
