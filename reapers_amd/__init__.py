@@ -155,7 +155,7 @@ def host():
             "rph_state_evolve_sched": ([_vp, _vp, _vp, _dbl, _dbl, _c.c_int, _dbl, _c.c_int,
                                         _c.c_int, _c.POINTER(EvolveStats)], _c.c_int),
             "rph_otoc": ([_vp, _vp, _vp, _vp, _dbl, _vp, _c.c_int, _c.c_int, _dbl, _vp,
-                          _c.POINTER(EvolveStats)], _c.c_int),
+                          _c.POINTER(EvolveStats), _vp], _c.c_int),
             "rph_batch_new": ([_c.c_int, _c.c_int, _vp, _vp, _vp, _vp, _c.c_int, _c.c_int,
                                _c.c_int], _vp),
             "rph_batch_free": ([_vp], None),
@@ -193,7 +193,7 @@ def host():
             "rph_syk_block_npoint": ([_c.c_int, _c.c_int, _c.c_int, _dbl, _dbl, _c.c_uint32,
                                       _c.c_int, _c.c_int, _vp, _u64, _dbl, _dbl, _c.c_int,
                                       _c.c_int, _dbl, _vp, _c.POINTER(_dbl),
-                                      _c.POINTER(EvolveStats)], _c.c_int),
+                                      _c.POINTER(EvolveStats), _vp], _c.c_int),
             "rph_state_evolve_ed": ([_vp, _vp, _dbl, _dbl], _c.c_int),
             "rph_state_ground_ed": ([_vp, _vp, _c.POINTER(_dbl)], _c.c_int),
             "rph_ham_dense": ([_vp, _c.c_int, _dbl, _dbl, _vp, _vp], _c.c_int),
@@ -453,14 +453,17 @@ class DeviceState:
                                              max_iters, reorth, _c.byref(st)))
         return st
 
-    def otoc(self, ham, A, B, times, beta=0.0, krydim=12, tol=0.0):
+    def otoc(self, ham, A, B, times, beta=0.0, krydim=12, tol=0.0, schedule=None):
         """F(t) = <i|A(t) B A(t) B|i> for each t, |i> this state (which
-        becomes exp(-beta H/8)|psi> when beta != 0).  Returns (F, stats)."""
+        becomes exp(-beta H/8)|psi> when beta != 0).  A Schedule chooses each
+        restart's Krylov dimension (krydim is then the fallback).  Returns
+        (F, stats)."""
         ts = np.ascontiguousarray(times, dtype=np.float64)
         out = np.zeros(2 * len(ts))
         st = EvolveStats()
         hcheck(host().rph_otoc(ham._h, A._h, B._h, self._s, beta, _ptr(ts), len(ts), krydim,
-                               tol, _ptr(out), _c.byref(st)))
+                               tol, _ptr(out), _c.byref(st),
+                               None if schedule is None else schedule._h))
         return out[0::2] + 1j * out[1::2], st
 
     def inner(self, other):
@@ -583,12 +586,13 @@ class BlockState:
 
 def syk_block_npoint(N, kind="otoc", sp=0.0, J=1.0, seed=0, regularize=False, otoc_idx=-1,
                      init=None, state_seed=1, beta=0.0, tmax=1.0, nsteps=10, krylov_dim=5,
-                     tol=0.0, fp=32):
+                     tol=0.0, fp=32, schedule=None):
     """One disorder realization of the parity-block SYK OTOC or Green's
     function: returns (t, v, e0, stats) with v_k / v_0 at t_k = k tmax/nsteps
     and e0 the ground-state energy the Hamiltonian was shifted by.  kind
     'otoc-trace' / 'green-trace' use the trace definitions by exact
-    diagonalization (no state; e0 = 0)."""
+    diagonalization (no state; e0 = 0).  A Schedule chooses each restart's
+    Krylov dimension (krylov_dim stays the fallback and the Lanczos size)."""
     k = {"otoc": 0, "green": 1, "otoc-trace": 2, "green-trace": 3}[kind]
     out = np.zeros(2 * (nsteps + 1))
     e0 = _dbl()
@@ -600,7 +604,7 @@ def syk_block_npoint(N, kind="otoc", sp=0.0, J=1.0, seed=0, regularize=False, ot
     hcheck(host().rph_syk_block_npoint(k, fp, N, sp, J, seed, int(regularize), otoc_idx,
                                        None if a is None else _ptr(a), state_seed, beta, tmax,
                                        nsteps, krylov_dim, tol, _ptr(out), _c.byref(e0),
-                                       _c.byref(st)))
+                                       _c.byref(st), None if schedule is None else schedule._h))
     t = np.arange(nsteps + 1) * (tmax / nsteps if nsteps else 0.0)
     return t, out[0::2] + 1j * out[1::2], e0.value, st
 

@@ -420,9 +420,11 @@ int rph_state_evolve_list(void *sp, void *hp, double t, double beta, const int *
 }
 
 // OTOC F(t_k) = <i|A(t)BA(t)B|i> of the state handle sp (which becomes
-// exp(-beta H/8)|psi> if beta != 0); out[2k], out[2k+1] = Re, Im F(t_k)
+// exp(-beta H/8)|psi> if beta != 0); out[2k], out[2k+1] = Re, Im F(t_k);
+// sched (nullable) chooses each restart's Krylov dimension, krydim the
+// fallback
 int rph_otoc(void *hp, void *ap, void *bp, void *sp, double beta, const double *times,
-             int ntimes, int krydim, double tol, double *out, rph_stats *stats)
+             int ntimes, int krydim, double tol, double *out, rph_stats *stats, void *sched)
 {
     return guarded([&] {
         Ham *h = (Ham *)hp, *a = (Ham *)ap, *b = (Ham *)bp;
@@ -434,6 +436,10 @@ int rph_otoc(void *hp, void *ap, void *bp, void *sp, double beta, const double *
         KrylovOptions opt;
         opt.krydim = krydim;
         opt.tol = tol;
+        if (sched) {
+            opt.schedule = (Sched *)sched;
+            opt.max_krydim = std::max(krydim, ((Sched *)sched)->max_dim);
+        }
         std::vector<double> ts(times, times + ntimes);
         KrylovStats st;
         auto t0 = std::chrono::steady_clock::now();
@@ -732,7 +738,7 @@ template<RealScalar Fp>
 void syk_block_npoint(int kind, int N, double sp, double J, uint32_t seed, int regularize,
                       int otoc_idx, const void *init, uint64_t state_seed, double beta,
                       double tmax, int nsteps, int krylov_dim, double tol, double *out,
-                      double *e0, KrylovStats &st)
+                      double *e0, KrylovStats &st, void *sched)
 {
     SYKBlockForm<Fp, GPUImpl> syk(N, (Fp)sp, (Fp)J);
     std::mt19937 rg(seed);
@@ -756,6 +762,10 @@ void syk_block_npoint(int kind, int N, double sp, double J, uint32_t seed, int r
     KrylovOptions opt;
     opt.krydim = krylov_dim;
     opt.tol = tol;
+    if (sched) {
+        opt.schedule = (Sched *)sched;
+        opt.max_krydim = std::max(krylov_dim, ((Sched *)sched)->max_dim);
+    }
     auto v = npoint_block<Fp, GPUImpl>(kind ? NPoint::Green : NPoint::Otoc, ham, op, s0, tmax,
                                        nsteps, beta, krylov_dim, opt, st, e0);
     for (size_t k = 0; k < v.size(); k++) {
@@ -976,11 +986,13 @@ int rph_block_state_ground(void *sp, void *LL, void *RR, int krydim, double eps,
 // the trace definitions by exact diagonalization (kind 2 = OTOC, 3 = Green's
 // function; no state, e0 = 0).
 // out receives 2 (nsteps + 1) doubles (Re, Im of v_k / v_0), e0 the
-// ground-state energy the Hamiltonian was shifted by.
+// ground-state energy the Hamiltonian was shifted by; sched (nullable) is a
+// schedule handle choosing each restart's Krylov dimension (krylov_dim is
+// then the fallback; the Lanczos E0 always uses krylov_dim).
 int rph_syk_block_npoint(int kind, int fp, int N, double sp, double J, uint32_t seed,
                          int regularize, int otoc_idx, const void *init, uint64_t state_seed,
                          double beta, double tmax, int nsteps, int krylov_dim, double tol,
-                         double *out, double *e0, rph_stats *stats)
+                         double *out, double *e0, rph_stats *stats, void *sched)
 {
     return guarded([&] {
         if (kind < 0 || kind > 3) ThrowException(InvalidArgument, "kind", "must be 0..3");
@@ -989,10 +1001,10 @@ int rph_syk_block_npoint(int kind, int fp, int N, double sp, double J, uint32_t 
         auto t0 = std::chrono::steady_clock::now();
         if (fp == 32)
             syk_block_npoint<float>(kind, N, sp, J, seed, regularize, otoc_idx, init, state_seed,
-                                    beta, tmax, nsteps, krylov_dim, tol, out, e0, st);
+                                    beta, tmax, nsteps, krylov_dim, tol, out, e0, st, sched);
         else
             syk_block_npoint<double>(kind, N, sp, J, seed, regularize, otoc_idx, init, state_seed,
-                                     beta, tmax, nsteps, krylov_dim, tol, out, e0, st);
+                                     beta, tmax, nsteps, krylov_dim, tol, out, e0, st, sched);
         GPUImpl::synchronize();
         double wall = std::chrono::duration<double, std::milli>(
                           std::chrono::steady_clock::now() - t0).count();

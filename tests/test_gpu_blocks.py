@@ -222,3 +222,21 @@ def test_syk_block_npoint_large(gpu):
                 vs.append(v)
             if len(vs) == 2:
                 assert np.abs(vs[0] - vs[1]).max() < 0.02
+
+
+def test_syk_block_npoint_with_proposer_vs_ed(gpu, orc):
+    from reapers_amd import proposer
+    N, seed, L, beta = 14, 12, 6, 2.0
+    s0 = random_state(np.random.default_rng(2), L)
+    sched = gpu.Schedule.from_mlp(proposer.load_weights())
+    t, v, e0, st = gpu.syk_block_npoint(N, "otoc", seed=seed, init=s0, beta=beta, tmax=5.0,
+                                        nsteps=5, krylov_dim=12, tol=1e-13, fp=64,
+                                        schedule=sched)
+    mLL = orc.dense_matrix(L, *orc.syk_block_terms(N, 0, seed=seed, fp=64))
+    mRR = orc.dense_matrix(L, *orc.syk_block_terms(N, 1, seed=seed, fp=64))
+    ob, oc = orc.majorana_block(N, N - 2, 0, fp=64)
+    mop = orc.dense_matrix(L, np.array([ob], np.uint64), np.array([oc]))
+    ref = ed_npoint("otoc", mLL, mRR, mop, s0, 5.0, 5, beta)
+    print("block OTOC with the proposer: %.2e, matvecs %d" % (np.abs(v - ref).max(), st.matvecs),
+          flush=True)
+    assert np.abs(v - ref).max() < 1e-10

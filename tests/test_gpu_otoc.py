@@ -77,3 +77,27 @@ def test_otoc_fp32_vs_fp64(gpu, orc):
     print("fp32 vs fp64 OTOC: max |F32 - F64| / |F(0)| = %.2e" % (err / 0.25), flush=True)
     # relative to |F(0)| = 1/4, the reference's normalization (examples/syk.cc:230-235)
     assert err / 0.25 < 1e-5
+
+
+def test_otoc_with_proposer_schedule_vs_ed(gpu, orc):
+    """The OTOC with the trained proposer choosing each restart's Krylov
+    dimension agrees with ED like the fixed dimension does."""
+    from reapers_amd import proposer
+    N, L, beta = 16, 8, 1.0
+    bits, coeff = orc.syk_terms(N, 0.0, seed=57, fp=64)
+    psi = random_state(np.random.default_rng(8), L)
+    times = np.array([0.0, 0.3, 2.0, 6.0])
+    ham = gpu.Hamiltonian(L, bits, coeff, fp=64)
+    A, (ab, ac) = majorana_ham(gpu, N, 1, L, 64)
+    B, (bb, bc) = majorana_ham(gpu, N, 2, L, 64)
+    sched = gpu.Schedule.from_mlp(proposer.load_weights())
+    s = gpu.DeviceState(L, fp=64)
+    s.load(psi)
+    F, st = s.otoc(ham, A, B, times, beta=beta, krydim=12, tol=1e-13, schedule=sched)
+    H = orc.dense_matrix(L, bits, coeff)
+    Am = orc.dense_matrix(L, np.array([ab], np.uint64), np.array([ac]))
+    Bm = orc.dense_matrix(L, np.array([bb], np.uint64), np.array([bc]))
+    ref = ed_otoc(H, Am, Bm, psi, times, beta)
+    print("OTOC with the proposer: max |F - F_ed| = %.2e, matvecs %d, rejected %d" % (
+        np.abs(F - ref).max(), st.matvecs, st.rejected), flush=True)
+    assert np.abs(F - ref).max() < 1e-11
