@@ -198,6 +198,8 @@ def host():
             "rph_state_ground_ed": ([_vp, _vp, _c.POINTER(_dbl)], _c.c_int),
             "rph_ham_dense": ([_vp, _c.c_int, _dbl, _dbl, _vp, _vp], _c.c_int),
             "rph_ham_matexp_entries": ([_vp, _c.POINTER(_i64)], _c.c_int),
+            "rph_state_set": ([_vp, _u64, _dbl, _dbl], _c.c_int),
+            "rph_cache_budget": ([_i64, _c.POINTER(_i64), _c.POINTER(_i64)], _c.c_int),
             "rph_ham_to_string": ([_c.c_int, _vp, _vp, _i64, _c.c_char_p, _i64,
                                    _c.POINTER(_i64)], _c.c_int),
             "rph_syk_block_dump": ([_c.c_int, _c.c_int, _dbl, _dbl, _c.c_uint32, _c.c_int,
@@ -338,6 +340,15 @@ def init(device=0):
     check(hip().rp_init(device))
 
 
+def matexp_cache_budget(max_bytes=None):
+    """(max, used) bytes of the matexp cache budget; max_bytes sets the
+    maximum first (GPUImpl::set_max_cache_size)."""
+    mx, used = _i64(), _i64()
+    hcheck(host().rph_cache_budget(-1 if max_bytes is None else int(max_bytes), _c.byref(mx),
+                                   _c.byref(used)))
+    return mx.value, used.value
+
+
 class Hamiltonian:
     """A REAPERS SumOps<Fp> with its device layout (C++ API handle)."""
 
@@ -423,6 +434,11 @@ class DeviceState:
 
     def random(self, seed):
         hcheck(host().rph_state_random(self._s, seed))
+
+    def __setitem__(self, i, value):
+        """One amplitude written on the device (State's ElemRefType)."""
+        value = complex(value)
+        hcheck(host().rph_state_set(self._s, int(i), value.real, value.imag))
 
     def copy_from(self, other, rounding=RP_ROUND_NEAREST):
         hcheck(host().rph_state_copy(self._s, other._s, rounding))

@@ -906,7 +906,9 @@ int rph_block_state_get(void *sp, uint64_t i, double *re, double *im)
 {
     return guarded([&] {
         BlkSt *s = (BlkSt *)sp;
-        auto c = s->fp == 32 ? std::complex<double>((*s->f)[i]) : (*s->d)[i];
+        auto c = s->fp == 32
+                     ? std::complex<double>(static_cast<const BlockState<float, GPUImpl> &>(*s->f)[i])
+                     : static_cast<const BlockState<double, GPUImpl> &>(*s->d)[i];
         *re = c.real();
         *im = c.imag();
     });
@@ -1147,6 +1149,32 @@ int rph_format_results(double dt, const double *v, int64_t n, char *buf, int64_t
         std::stringstream ss;
         write_results(ss, dt, c);
         put_string(ss.str(), buf, cap, len);
+    });
+}
+
+}  // extern "C"
+
+// element access and the matexp cache budget -----------------------------------
+
+extern "C" {
+
+// st[i] = re + i im through State's writable element (ElemRefType)
+int rph_state_set(void *sp, uint64_t i, double re, double im)
+{
+    return guarded([&] {
+        St *s = (St *)sp;
+        if (s->fp == 32) (*s->f)[i] = complex<float>{(float)re, (float)im};
+        else (*s->d)[i] = complex<double>{re, im};
+    });
+}
+
+// GPUImpl::set_max_cache_size (bytes >= 0) and the budget's state
+int rph_cache_budget(int64_t set_bytes, int64_t *max_bytes, int64_t *used_bytes)
+{
+    return guarded([&] {
+        if (set_bytes >= 0) GPUImpl::set_max_cache_size((size_t)set_bytes);
+        *max_bytes = (int64_t)GPUImpl::get_max_cache_size();
+        *used_bytes = (int64_t)GPUImpl::get_current_cache_size();
     });
 }
 

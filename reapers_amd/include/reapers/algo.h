@@ -298,7 +298,7 @@ public:
         auto &&es = ham.get_eigensystem(len);
         const auto &vals = std::get<0>(es);
         size_t idx = std::min_element(vals.begin(), vals.end()) - vals.begin();
-        Impl::copy_column(s.dim(), s.buf(), std::get<1>(es), idx);
+        Impl::copy_vec(s.dim(), s.buf(), std::get<1>(es), idx);
         return vals[idx];
     }
 
@@ -307,7 +307,7 @@ public:
                        FpType beta = 0.0, Args &&...) {
         s.enlarge(2);
         const auto &m = ops.matexp(complex<FpType>{-beta, -t}, s.spin_chain_length());
-        Impl::mat_vec(s.dim(), s.buf(1), m, s.buf());
+        Impl::mat_mul(s.dim(), s.buf(1), m, s.buf());
         s.inc_curbuf(1);
     }
 };

@@ -8,11 +8,12 @@
 // on the device from the term list, the eigensystem comes from rocSOLVER's
 // heevd, and exp(c H) = V diag(exp(c w)) V^H is formed by one column-scaling
 // kernel and one GEMM.  Results of matexp are kept in a cache per operator
-// keyed by c, under a byte budget shared by all operators (half of the
-// device memory by default, as the reference's cache_reserve,
-// inc/gpuimpl.h:560-580); when an insertion exceeds it, the least-used
-// entries of that cache are dropped (inc/cpuimpl.h:29-88).  Every cached
-// form is dropped when the term list changes.
+// keyed by c, under the byte budget GPUImpl::cache_reserve keeps for all
+// operators (half of the free device memory by default,
+// GPUImpl::set_max_cache_size changes it; inc/gpuimpl.h:560-597); when an
+// insertion exceeds it, the least-used entries of that cache are dropped
+// (inc/cpuimpl.h:29-88).  Every cached form is dropped when the term list
+// changes.
 //
 // This is the small-N path (n = 2^len <= 2^15): it serves as an on-device
 // check of the Krylov path and computes trace-definition correlators.
@@ -205,23 +206,6 @@ public:
 
 namespace internal {
 
-// bytes held by all matexp caches, against a budget
-struct MatexpBudget {
-    size_t used = 0;
-    size_t limit = 0;
-};
-
-inline MatexpBudget &matexp_budget() {
-    static MatexpBudget b;
-    if (!b.limit) {
-        size_t fr = 0, tot = 0;
-        GPUImpl::ensure_context();
-        if (rp_mem_get_info(&fr, &tot) == RP_OK && tot) b.limit = tot / 2;
-        else b.limit = (size_t)1 << 34;
-    }
-    return b;
-}
-
 // matexp results of one operator keyed by c, with their use counts
 template<RealScalar FpType>
 class MatexpCache {
@@ -232,7 +216,7 @@ class MatexpCache {
     std::map<std::pair<FpType, FpType>, Entry> cache;
 
     void drop(typename std::map<std::pair<FpType, FpType>, Entry>::iterator it) {
-        matexp_budget().used -= it->second.m.bytes();
+        GPUImpl::cache_release(it->second.m.bytes());
         cache.erase(it);
     }
 
@@ -249,16 +233,21 @@ public:
         return &it->second.m;
     }
 
+    // the least-used entries are dropped until the result fits the budget
+    // (or the cache is empty: a result larger than the budget is kept alone)
     const DevMatrix<FpType> &insert(complex<FpType> c, DevMatrix<FpType> &&m) {
-        auto &b = matexp_budget();
         size_t sz = m.bytes();
-        while (!cache.empty() && b.used + sz > b.limit) {
+        while (!GPUImpl::cache_reserve(sz)) {
+            if (cache.empty()) {
+                GPUImpl::set_max_cache_size(std::max(GPUImpl::get_max_cache_size(),
+                                                     GPUImpl::get_current_cache_size() + sz));
+                continue;
+            }
             auto victim = cache.begin();
             for (auto it = cache.begin(); it != cache.end(); ++it)
                 if (it->second.hits < victim->second.hits) victim = it;
             drop(victim);
         }
-        b.used += sz;
         auto it = cache.insert_or_assign({c.real(), c.imag()}, Entry{std::move(m), 0}).first;
         return it->second.m;
     }

@@ -160,6 +160,22 @@ public:
             RP_CALL(rp_memcpy(&c, (const complex<FpType> *)ptr + idx, sizeof(c), RP_D2H));
             return c;
         }
+        // a writable element (DeviceVec::ElemRefType, inc/gpuimpl.h:413-424):
+        // assignment copies one value to the device, reading copies it back
+        class ElemRefType {
+            DeviceVec &vec;
+            size_t idx;
+        public:
+            ElemRefType(DeviceVec &v, size_t i) : vec(v), idx(i) {}
+            ElemRefType &operator=(complex<FpType> s) {
+                RP_CALL(rp_memcpy((complex<FpType> *)vec.ptr + idx, &s, sizeof(s), RP_H2D));
+                return *this;
+            }
+            operator complex<FpType>() const {
+                return static_cast<const DeviceVec &>(vec)[idx];
+            }
+        };
+        ElemRefType operator[](size_t idx) { return ElemRefType(*this, idx); }
         // host <-> device transfers of the whole vector
         void upload(const complex<FpType> *host) {
             RP_CALL(rp_memcpy(ptr, host, dim * sizeof(complex<FpType>), RP_H2D));
@@ -174,6 +190,46 @@ public:
     template<RealScalar FpType> using BufType = DeviceVec<FpType> &;
     template<RealScalar FpType> using ConstBufType = const DeviceVec<FpType> &;
     template<RealScalar FpType> using SumOps = DevSumOps<FpType>;
+    template<RealScalar FpType> using ElemRefType = typename DeviceVec<FpType>::ElemRefType;
+
+private:
+    struct CacheBudget {
+        size_t max = 0, used = 0;
+        bool set = false;
+    };
+    static CacheBudget &cache_budget() {
+        static CacheBudget b;
+        if (!b.set) {
+            ensure_context();
+            size_t fr = 0, tot = 0;
+            b.max = rp_mem_get_info(&fr, &tot) == RP_OK ? fr / 2 : ((size_t)1 << 34);
+            b.set = true;
+        }
+        return b;
+    }
+
+public:
+    // the matexp cache budget shared by all operators (inc/gpuimpl.h:
+    // 560-597): half of the free device memory at first use by default
+    static bool cache_reserve(size_t sz) {
+        auto &b = cache_budget();
+        if (b.used + sz > b.max) return false;
+        b.used += sz;
+        return true;
+    }
+    static void cache_release(size_t sz) {
+        auto &b = cache_budget();
+        b.used -= std::min(sz, b.used);
+    }
+    static void set_max_cache_size(size_t sz) { cache_budget().max = sz; }
+    static void set_max_cache_size(float fraction) {
+        size_t fr = 0, tot = 0;
+        ensure_context();
+        RP_CALL(rp_mem_get_info(&fr, &tot));
+        cache_budget().max = (size_t)(fr * (double)fraction);
+    }
+    static size_t get_max_cache_size() { return cache_budget().max; }
+    static size_t get_current_cache_size() { return cache_budget().used; }
 
     static void synchronize() { RP_CALL(rp_synchronize()); }
 
@@ -322,16 +378,18 @@ public:
     // res = mat * vec for a dim x dim device matrix (mat_mul,
     // inc/gpuimpl.h:1009-1022); res and vec must differ
     template<RealScalar FpType, typename Mat>
-    static void mat_vec(size_t dim, BufType<FpType> res, const Mat &mat, ConstBufType<FpType> vec) {
+    static void mat_mul(size_t dim, BufType<FpType> res, const Mat &mat, ConstBufType<FpType> vec) {
         if (mat.rows() != dim || mat.cols() != dim)
             ThrowException(InvalidArgument, "matrix dimensions", "must match the state");
         RP_CALL(rp_dense_gemm(internal::rp_prec<FpType>(), dim, 1, dim, 0, 0, 1.0, 0.0,
                               mat.data(), dim, vec.ptr, dim, 0.0, 0.0, res.ptr, dim));
     }
 
-    // v = column idx of a dim x dim device matrix
+    // v = column idx of a dim x dim device matrix (copy_vec,
+    // inc/gpuimpl.h:891-906)
     template<RealScalar FpType, typename Mat>
-    static void copy_column(size_t dim, BufType<FpType> v, const Mat &mat, size_t idx) {
+        requires requires(const Mat &m) { m.rows(); m.cols(); m.data(); }
+    static void copy_vec(size_t dim, BufType<FpType> v, const Mat &mat, size_t idx) {
         if (mat.rows() != dim || idx >= mat.cols())
             ThrowException(InvalidArgument, "column", "out of range");
         RP_CALL(rp_memcpy(v.ptr, (const complex<FpType> *)mat.data() + idx * dim,

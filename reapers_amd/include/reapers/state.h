@@ -164,6 +164,13 @@ public:
         return buf()[i];
     }
 
+    // a writable amplitude (inc/state.h:225-237): st[i] = c copies one value
+    // to the device
+    typename Impl::template ElemRefType<FpType> operator[](IndexType i) {
+        if (i >= dim()) ThrowException(StateIndexTooLarge, i, dim() - 1);
+        return buf()[i];
+    }
+
     // whole-vector host transfers (interleaved complex<FpType>)
     void load(const complex<FpType> *host) { buf().upload(host); }
     void store(complex<FpType> *host) const { buf().download(host); }
@@ -399,6 +406,14 @@ public:
         size_t half = dim() / 2;
         if (i < half) return this->nullL ? complex<FpType>{} : this->L[i];
         return this->nullR ? complex<FpType>{} : this->R[i - half];
+    }
+
+    // a writable amplitude; both sectors must be defined (inc/state.h:542-550)
+    typename Impl::template ElemRefType<FpType> operator[](size_t i) {
+        if (this->nullL || this->nullR)
+            ThrowException(InvalidArgument, "Both parity blocks must be defined.");
+        size_t half = dim() / 2;
+        return i < half ? this->L[i] : this->R[i - half];
     }
 
     double norm64() const {

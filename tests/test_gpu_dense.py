@@ -159,3 +159,41 @@ def test_trace_and_state_definitions_agree_statistically(gpu):
                                        krylov_dim=20, tol=1e-12, fp=64)
     print("green trace %s\ngreen state %s" % (vt, vs), flush=True)
     assert np.abs(vt - vs).max() < 0.1
+
+
+def test_matexp_cache_budget_evicts(gpu, orc):
+    """With a budget of three 2^8 x 2^8 complex<double> matrices, a fourth
+    matexp drops the least-used entry; results stay correct."""
+    L = 8
+    bits, coeff = orc.syk_block_terms(18, 0, seed=3, fp=64)
+    h = gpu.Hamiltonian(L, bits, coeff, fp=64)
+    H = orc.dense_matrix(L, bits, coeff)
+    old_max, _ = gpu.matexp_cache_budget()
+    one = (1 << L) ** 2 * 16
+    try:
+        _, used0 = gpu.matexp_cache_budget()
+        gpu.matexp_cache_budget(used0 + 3 * one)
+        for k, t in enumerate([0.1, 0.2, 0.3]):
+            h.matexp(-1j * t)
+        h.matexp(-0.1j)          # one more use of t = 0.1
+        assert h.matexp_cache_entries() == 3
+        E = h.matexp(-0.4j)      # evicts the least used (t = 0.2 or 0.3)
+        assert h.matexp_cache_entries() == 3
+        mx, used = gpu.matexp_cache_budget()
+        assert used <= mx
+        assert np.abs(E - scipy.linalg.expm(-0.4j * H)).max() < 1e-12
+    finally:
+        gpu.matexp_cache_budget(old_max)
+
+
+@pytest.mark.parametrize("fp", [32, 64])
+def test_state_element_write(gpu, fp):
+    L = 6
+    s = gpu.DeviceState(L, fp=fp)
+    s.random(3)
+    v = s.store()
+    s[5] = 0.25 - 0.5j
+    s[63] = 2.0
+    w = s.store()
+    v[5], v[63] = 0.25 - 0.5j, 2.0
+    assert np.array_equal(w, v.astype(w.dtype))
