@@ -248,6 +248,9 @@ int rp_prof_query(int kernel_class, uint64_t *launches, double *total_ms);
 typedef struct rp_batch *rp_batch_t;
 typedef struct rp_graph *rp_graph_t;
 #define RP_BATCH_AUTOTUNE 1   /* time the instantiated tilings, keep the fastest */
+/* complex<float> batches: the terms of each pair of tile groups are summed in
+ * FP32 and those sums in FP64 (as RP_TERMS_ACC_GROUP; others sum in FP64) */
+#define RP_BATCH_ACC_GROUP 2
 /* force the apply's tile bits b: the apply is instantiated for tiles of
  * 32 KiB and 64 KiB, b = log2(32 KiB / (nrhs * element bytes)) or b + 1 */
 #define RP_BATCH_TILE_BITS(b) ((b) << 8)

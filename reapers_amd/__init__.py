@@ -751,7 +751,8 @@ class Batch:
     """BatchedStates<Fp>: problems (lens[p], bits[p], coeff[p]) with nrhs
     states each in the block-state layout."""
 
-    def __init__(self, lens, terms, nrhs=4, kmax=20, fp=32, autotune=True, tile_bits=0):
+    def __init__(self, lens, terms, nrhs=4, kmax=20, fp=32, autotune=True, tile_bits=0,
+                 acc_group=False):
         self.lens = [int(x) for x in lens]
         self.nprob = len(self.lens)
         self.nrhs = nrhs
@@ -763,8 +764,9 @@ class Batch:
         toff[1:] = np.cumsum([len(b) for b, _ in terms])
         la = np.array(self.lens, np.int32)
         self._h = host().rph_batch_new(fp, self.nprob, _ptr(la), _ptr(bits), _ptr(co), _ptr(toff),
-                                       nrhs, kmax, (tile_bits << 8) if tile_bits
-                                       else (1 if autotune else 0))
+                                       nrhs, kmax, ((tile_bits << 8) if tile_bits
+                                                    else (1 if autotune else 0))
+                                       | (2 if acc_group else 0))
         if not self._h:
             raise RpError(-1, host().rph_last_error().decode())
         self.info = BatchInfo()

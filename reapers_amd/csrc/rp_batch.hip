@@ -32,6 +32,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 namespace rp {
@@ -50,6 +51,7 @@ struct rp_batch {
     int prec = RP_C64;
     int nprob = 0, ns = 1, kmax = 0;
     int tile_bits = 0, threads = 0, chunk_bits = 0;
+    int group_acc = 0;                 // FP32 sums per pair of tile groups (complex<float>)
     uint64_t total_tiles = 0, total_chunks = 0, nbasis = 0;
     uint64_t nterms = 0, ngroups = 0;
     std::vector<int> len;
@@ -103,12 +105,29 @@ __device__ inline void dma_bytes(const char *__restrict__ g, char *l)
     }
 }
 
-template <typename R, typename A, int TB, int NT, int NS>
+// a sub's coefficient in the partial-sum type P (the float copies when FP32
+// partial sums are formed from an FP64-coefficient layout)
+template <typename P, typename A>
+__device__ __forceinline__ P bsub_re(const SubRec<A> &S)
+{
+    if constexpr (std::is_same_v<P, float> && !std::is_same_v<A, float>) return S.fr;
+    else return (P)S.cr;
+}
+template <typename P, typename A>
+__device__ __forceinline__ P bsub_im(const SubRec<A> &S)
+{
+    if constexpr (std::is_same_v<P, float> && !std::is_same_v<A, float>) return S.fi;
+    else return (P)S.ci;
+}
+
+// Apply every sub of group G from the staged tile `buf` to the RP rows x NS
+// states of this thread, summing in P.
+template <typename R, typename A, int TB, int NT, int NS, typename P = A>
 __device__ __forceinline__ void batch_group(const GroupRuns &G, const Elt<R, NS> *buf,
                                             const SubRec<A> *__restrict__ subs,
                                             const TermRec<A> *__restrict__ terms, uint32_t t0,
-                                            uint32_t tid, A (&ar)[(1 << TB) / NT][NS],
-                                            A (&ai)[(1 << TB) / NT][NS])
+                                            uint32_t tid, P (&ar)[(1 << TB) / NT][NS],
+                                            P (&ai)[(1 << TB) / NT][NS])
 {
     using E = Elt<R, NS>;
     constexpr int RP = (1 << TB) / NT;
@@ -118,15 +137,15 @@ __device__ __forceinline__ void batch_group(const GroupRuns &G, const Elt<R, NS>
         const SubRec<A> S = subs[s];
         const E *row = buf + (tid ^ (S.xl & (NT - 1)));
         const uint32_t xr = S.xl >> LB, zr = (S.z >> LB) & (RP - 1);
-        const A c = parity32((t0 | tid) & S.z) ? -S.cr : S.cr;
+        const P c0 = bsub_re<P>(S), c = parity32((t0 | tid) & S.z) ? -c0 : c0;
 #pragma unroll
         for (int r = 0; r < RP; r++) {
             const E e = row[(r ^ xr) * NT];
-            const A cc = parity32(r & zr) ? -c : c;
+            const P cc = parity32(r & zr) ? -c : c;
 #pragma unroll
             for (int q = 0; q < NS; q++) {
-                ar[r][q] += cc * (A)e.v[q].x;
-                ai[r][q] += cc * (A)e.v[q].y;
+                ar[r][q] += cc * (P)e.v[q].x;
+                ai[r][q] += cc * (P)e.v[q].y;
             }
         }
     }
@@ -135,15 +154,15 @@ __device__ __forceinline__ void batch_group(const GroupRuns &G, const Elt<R, NS>
         const SubRec<A> S = subs[s];
         const E *row = buf + (tid ^ (S.xl & (NT - 1)));
         const uint32_t xr = S.xl >> LB, zr = (S.z >> LB) & (RP - 1);
-        const A c = parity32((t0 | tid) & S.z) ? -S.ci : S.ci;
+        const P c0 = bsub_im<P>(S), c = parity32((t0 | tid) & S.z) ? -c0 : c0;
 #pragma unroll
         for (int r = 0; r < RP; r++) {
             const E e = row[(r ^ xr) * NT];
-            const A cc = parity32(r & zr) ? -c : c;
+            const P cc = parity32(r & zr) ? -c : c;
 #pragma unroll
             for (int q = 0; q < NS; q++) {
-                ar[r][q] -= cc * (A)e.v[q].y;
-                ai[r][q] += cc * (A)e.v[q].x;
+                ar[r][q] -= cc * (P)e.v[q].y;
+                ai[r][q] += cc * (P)e.v[q].x;
             }
         }
     }
@@ -152,20 +171,22 @@ __device__ __forceinline__ void batch_group(const GroupRuns &G, const Elt<R, NS>
         const E *row = buf + (tid ^ (S.xl & (NT - 1)));
         const uint32_t xr = S.xl >> LB, zr = (S.z >> LB) & (RP - 1);
         const bool ln = parity32((t0 | tid) & S.z);
-        const A cr = ln ? -S.cr : S.cr, ci = ln ? -S.ci : S.ci;
+        const P r0 = bsub_re<P>(S), i0 = bsub_im<P>(S);
+        const P cr = ln ? -r0 : r0, ci = ln ? -i0 : i0;
 #pragma unroll
         for (int r = 0; r < RP; r++) {
             const E e = row[(r ^ xr) * NT];
             const bool n = parity32(r & zr);
-            const A a = n ? -cr : cr, b = n ? -ci : ci;
+            const P a = n ? -cr : cr, b = n ? -ci : ci;
 #pragma unroll
             for (int q = 0; q < NS; q++) {
-                ar[r][q] += a * (A)e.v[q].x - b * (A)e.v[q].y;
-                ai[r][q] += a * (A)e.v[q].y + b * (A)e.v[q].x;
+                ar[r][q] += a * (P)e.v[q].x - b * (P)e.v[q].y;
+                ai[r][q] += a * (P)e.v[q].y + b * (P)e.v[q].x;
             }
         }
     }
-    // several terms share x: d(m) = sum_t c_t (-1)^popcount(m & z_t)
+    // several terms share x: d(m) = sum_t c_t (-1)^popcount(m & z_t), formed
+    // in A, applied in P
     for (uint32_t s = G.b[kRunMulti]; s < G.b[kRunMulti + 1]; s++) {
         const SubRec<A> S = subs[s];
         const E *row = buf + (tid ^ (S.xl & (NT - 1)));
@@ -179,17 +200,20 @@ __device__ __forceinline__ void batch_group(const GroupRuns &G, const Elt<R, NS>
                 dr += n ? -C.cr : C.cr;
                 di += n ? -C.ci : C.ci;
             }
+            const P pr = (P)dr, pi = (P)di;
             const E e = row[(r ^ xr) * NT];
 #pragma unroll
             for (int q = 0; q < NS; q++) {
-                ar[r][q] += dr * (A)e.v[q].x - di * (A)e.v[q].y;
-                ai[r][q] += dr * (A)e.v[q].y + di * (A)e.v[q].x;
+                ar[r][q] += pr * (P)e.v[q].x - pi * (P)e.v[q].y;
+                ai[r][q] += pr * (P)e.v[q].y + pi * (P)e.v[q].x;
             }
         }
     }
 }
 
-template <typename R, typename A, int TB, int NT, int NS>
+// GA: the terms of each pair of tile groups are summed in FP32 partials and
+// those added to the FP64 sums (complex<float> states with FP64 layouts)
+template <typename R, typename A, int TB, int NT, int NS, bool GA = false>
 __global__ void __launch_bounds__(NT, (batch_waves_per_eu<R, A, TB, NT, NS>()))
 k_batch_apply(const Elt<R, NS> *__restrict__ in, Elt<R, NS> *__restrict__ out,
               const ProbDesc *__restrict__ probs, const uint32_t *__restrict__ tile_prob,
@@ -226,10 +250,31 @@ k_batch_apply(const Elt<R, NS> *__restrict__ in, Elt<R, NS> *__restrict__ out,
         const E *pin = in + P.off;
         E *pout = out + P.off;
         A ar[RP][NS], ai[RP][NS];
+        float pr[RP][NS], pi[RP][NS];
 #pragma unroll
         for (int r = 0; r < RP; r++)
 #pragma unroll
-            for (int q = 0; q < NS; q++) { ar[r][q] = 0; ai[r][q] = 0; }
+            for (int q = 0; q < NS; q++) {
+                ar[r][q] = 0;
+                ai[r][q] = 0;
+                pr[r][q] = 0;
+                pi[r][q] = 0;
+            }
+        auto flush = [&]() {
+#pragma unroll
+            for (int r = 0; r < RP; r++)
+#pragma unroll
+                for (int q = 0; q < NS; q++) {
+                    ar[r][q] += (A)pr[r][q];
+                    ai[r][q] += (A)pi[r][q];
+                    pr[r][q] = 0;
+                    pi[r][q] = 0;
+                }
+        };
+        auto group = [&](const GroupRuns &Gr, const E *tile) {
+            if constexpr (GA) batch_group<R, A, TB, NT, NS, float>(Gr, tile, subs, terms, t0, tid, pr, pi);
+            else batch_group<R, A, TB, NT, NS>(Gr, tile, subs, terms, t0, tid, ar, ai);
+        };
         const int ng = (int)(P.g1 - P.g0);
         const GroupRuns *gr = groups + P.g0;
         // the previous item's reads of both tiles are complete
@@ -240,14 +285,16 @@ k_batch_apply(const Elt<R, NS> *__restrict__ in, Elt<R, NS> *__restrict__ out,
             __syncthreads();
             if (g + 1 < ng)
                 dma_bytes<BYTES, NT>((const char *)(pin + (t0 ^ (gr[g + 1].xh << TB))), (char *)tile1);
-            batch_group<R, A, TB, NT, NS>(gr[g], tile0, subs, terms, t0, tid, ar, ai);
+            group(gr[g], tile0);
             if (g + 1 >= ng) break;
             __builtin_amdgcn_s_waitcnt(0);
             __syncthreads();
             if (g + 2 < ng)
                 dma_bytes<BYTES, NT>((const char *)(pin + (t0 ^ (gr[g + 2].xh << TB))), (char *)tile0);
-            batch_group<R, A, TB, NT, NS>(gr[g + 1], tile1, subs, terms, t0, tid, ar, ai);
+            group(gr[g + 1], tile1);
+            if constexpr (GA) flush();
         }
+        if constexpr (GA) flush();
 #pragma unroll
         for (int r = 0; r < RP; r++) {
             const uint32_t m = t0 + r * NT + tid;
@@ -550,6 +597,16 @@ template <typename R, typename A, int TB, int NT, int NS>
 int launch_apply_t(rp_batch *b, void *res, const void *vec, int accumulate)
 {
     using E = Elt<R, NS>;
+    if constexpr (sizeof(R) == 4 && sizeof(A) == 8) {
+        if (b->group_acc) {
+            RP_LAUNCH(RP_KERNEL_APPLY, (k_batch_apply<R, A, TB, NT, NS, true>), dim3(b->grid),
+                      dim3(NT), 0, (const E *)vec, (E *)res, (const ProbDesc *)b->d_probs,
+                      (const uint32_t *)b->d_tile_prob, (uint32_t)b->total_tiles,
+                      (const GroupRuns *)b->d_groups, (const SubRec<A> *)b->d_subs,
+                      (const TermRec<A> *)b->d_terms, (const int *)b->d_active, accumulate);
+            return RP_OK;
+        }
+    }
     RP_LAUNCH(RP_KERNEL_APPLY, (k_batch_apply<R, A, TB, NT, NS>), dim3(b->grid), dim3(NT), 0,
               (const E *)vec, (E *)res, (const ProbDesc *)b->d_probs,
               (const uint32_t *)b->d_tile_prob, (uint32_t)b->total_tiles,
@@ -679,7 +736,7 @@ int rp_batch_create(rp_batch_t *out, int prec, int nprob, const int *len, const 
     RP_REQUIRE(len && term_off && (term_off[nprob] == 0 || (bits && coeff)), "null argument");
     RP_REQUIRE(nrhs == 1 || nrhs == 2 || nrhs == 4, "nrhs must be 1, 2 or 4");
     RP_REQUIRE(kmax >= 2 && kmax <= 64, "kmax must be in [2, 64]");
-    RP_REQUIRE((flags & ~(RP_BATCH_AUTOTUNE | 0xff00)) == 0, "unknown flags");
+    RP_REQUIRE((flags & ~(RP_BATCH_AUTOTUNE | RP_BATCH_ACC_GROUP | 0xff00)) == 0, "unknown flags");
     const int forced_tb = (flags >> 8) & 0xff;
     RP_REQUIRE(forced_tb == 0 || forced_tb == tile_bits_for(prec, nrhs, 0) ||
                    forced_tb == tile_bits_for(prec, nrhs, 1),
@@ -692,6 +749,7 @@ int rp_batch_create(rp_batch_t *out, int prec, int nprob, const int *len, const 
     }
     rp_batch *b = new rp_batch;
     b->prec = prec;
+    b->group_acc = (prec == RP_C64 && (flags & RP_BATCH_ACC_GROUP)) ? 1 : 0;
     b->nprob = nprob;
     b->ns = nrhs;
     b->kmax = kmax;

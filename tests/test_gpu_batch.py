@@ -150,3 +150,41 @@ def test_batch_with_proposer_schedule(gpu, orc):
     print("batch + proposer: err %.2e, matvecs %d, restarts %s, rejected %d" % (
         worst, st.matvecs, rs.max(1).tolist(), st.rejected), flush=True)
     assert worst < 10 * tol
+
+
+@pytest.mark.parametrize("nrhs", [1, 2, 4])
+@pytest.mark.parametrize("large", [0, 1])
+def test_batch_group_accumulation_vs_oracle(gpu, orc, nrhs, large):
+    """RP_BATCH_ACC_GROUP (complex<float>): FP32 sums per pair of tile groups,
+    FP64 across, against the oracle's FP64 apply; then an evolution."""
+    tb = tile_bits(32, nrhs, large)
+    rng = np.random.default_rng(700 + nrhs * 10 + tb)
+    lens = [14, 16, 15, 14] if tb > 12 else [12, 14, 13, 12]
+    terms = [orc.syk_terms(2 * L, 8.0, seed=800 + i, fp=32) for i, L in enumerate(lens)]
+    b = gpu.Batch(lens, terms, nrhs=nrhs, kmax=10, fp=32, tile_bits=tb, acc_group=True)
+    states = [np.stack([random_state(rng, L, np.complex64) for _ in range(nrhs)], 1) for L in lens]
+    b.load(states)
+    b.apply()
+    got = b.store()
+    worst = 0.0
+    for p, L in enumerate(lens):
+        bits, coeff = terms[p]
+        for s in range(nrhs):
+            ref = orc.apply(L, bits, coeff, states[p][:, s].astype(np.complex128), fp=64)
+            worst = max(worst, np.abs(got[p][:, s] - ref).max() / np.abs(ref).max())
+    print("acc group nrhs=%d tb=%d: apply max rel err %.2e" % (nrhs, tb, worst), flush=True)
+    # FP32 partial sums over a pair of groups: a few FP32 ulps of the largest
+    # amplitude, the bound of the FP64-accumulating batched apply above
+    assert worst < 2e-6
+    b.load(states)
+    st, rs = b.evolve(2.0, m=10, tol=1e-7)
+    got = b.store()
+    worst = 0.0
+    for p, L in enumerate(lens):
+        bits, coeff = terms[p]
+        for s in range(nrhs):
+            ref, _ = orc.krylov_evolve(L, bits, coeff, states[p][:, s].astype(np.complex128), 2.0,
+                                       0.0, krydim=20, tol=1e-14)
+            worst = max(worst, np.linalg.norm(got[p][:, s] - ref))
+    print("acc group nrhs=%d tb=%d: evolve max |err| %.2e" % (nrhs, tb, worst), flush=True)
+    assert worst < 1e-5
