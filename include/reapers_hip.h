@@ -251,6 +251,9 @@ typedef struct rp_graph *rp_graph_t;
 /* complex<float> batches: the terms of each pair of tile groups are summed in
  * FP32 and those sums in FP64 (as RP_TERMS_ACC_GROUP; others sum in FP64) */
 #define RP_BATCH_ACC_GROUP 2
+/* with the 32 KiB tile: 512 threads (2 rows each) instead of 256 (4 rows);
+ * the autotuner also times this variant */
+#define RP_BATCH_WIDE 4
 /* force the apply's tile bits b: the apply is instantiated for tiles of
  * 32 KiB and 64 KiB, b = log2(32 KiB / (nrhs * element bytes)) or b + 1 */
 #define RP_BATCH_TILE_BITS(b) ((b) << 8)

@@ -752,7 +752,7 @@ class Batch:
     states each in the block-state layout."""
 
     def __init__(self, lens, terms, nrhs=4, kmax=20, fp=32, autotune=True, tile_bits=0,
-                 acc_group=False):
+                 acc_group=False, wide=False):
         self.lens = [int(x) for x in lens]
         self.nprob = len(self.lens)
         self.nrhs = nrhs
@@ -766,7 +766,7 @@ class Batch:
         self._h = host().rph_batch_new(fp, self.nprob, _ptr(la), _ptr(bits), _ptr(co), _ptr(toff),
                                        nrhs, kmax, ((tile_bits << 8) if tile_bits
                                                     else (1 if autotune else 0))
-                                       | (2 if acc_group else 0))
+                                       | (2 if acc_group else 0) | (4 if wide else 0))
         if not self._h:
             raise RpError(-1, host().rph_last_error().decode())
         self.info = BatchInfo()

@@ -619,7 +619,10 @@ template <typename R, typename A, int NS>
 int launch_apply_ns(rp_batch *b, void *res, const void *vec, int accumulate)
 {
     constexpr int S = small_tb<R, NS>();
-    if (b->tile_bits == S) return launch_apply_t<R, A, S, 256, NS>(b, res, vec, accumulate);
+    if (b->tile_bits == S && b->threads == 256)
+        return launch_apply_t<R, A, S, 256, NS>(b, res, vec, accumulate);
+    if (b->tile_bits == S && b->threads == 512)
+        return launch_apply_t<R, A, S, 512, NS>(b, res, vec, accumulate);
     if (b->tile_bits == S + 1) return launch_apply_t<R, A, S + 1, 512, NS>(b, res, vec, accumulate);
     return set_error(RP_ERR_RUNTIME, "batched apply: tiling b=%d not instantiated for %d states",
                      b->tile_bits, NS);
@@ -736,7 +739,8 @@ int rp_batch_create(rp_batch_t *out, int prec, int nprob, const int *len, const 
     RP_REQUIRE(len && term_off && (term_off[nprob] == 0 || (bits && coeff)), "null argument");
     RP_REQUIRE(nrhs == 1 || nrhs == 2 || nrhs == 4, "nrhs must be 1, 2 or 4");
     RP_REQUIRE(kmax >= 2 && kmax <= 64, "kmax must be in [2, 64]");
-    RP_REQUIRE((flags & ~(RP_BATCH_AUTOTUNE | RP_BATCH_ACC_GROUP | 0xff00)) == 0, "unknown flags");
+    RP_REQUIRE((flags & ~(RP_BATCH_AUTOTUNE | RP_BATCH_ACC_GROUP | RP_BATCH_WIDE | 0xff00)) == 0,
+               "unknown flags");
     const int forced_tb = (flags >> 8) & 0xff;
     RP_REQUIRE(forced_tb == 0 || forced_tb == tile_bits_for(prec, nrhs, 0) ||
                    forced_tb == tile_bits_for(prec, nrhs, 1),
@@ -794,12 +798,13 @@ int rp_batch_create(rp_batch_t *out, int prec, int nprob, const int *len, const 
                                 hipMemcpyHostToDevice, c.stream));
     RP_CHECK_HIP(hipStreamSynchronize(c.stream));
     b->V.assign(kmax + 2, nullptr);
-    // default tiling: 2^10-amplitude tiles, 256 threads
-    // default tiling: 64 KiB tiles if every problem has that many amplitudes
+    // default tiling: 64 KiB tiles if every problem has that many amplitudes;
+    // RP_BATCH_TILE_BITS of the 32 KiB tile forces that (256 threads, or 512
+    // with RP_BATCH_WIDE)
     int tb = tile_bits_for(prec, nrhs, 1), nt = 512;
     if (tb > minlen || forced_tb == tile_bits_for(prec, nrhs, 0)) {
         tb = tile_bits_for(prec, nrhs, 0);
-        nt = 256;
+        nt = (flags & RP_BATCH_WIDE) ? 512 : 256;
     }
     if (tb > minlen) {
         batch_release(b);
@@ -826,8 +831,12 @@ int rp_batch_create(rp_batch_t *out, int prec, int nprob, const int *len, const 
         float best = 1e30f;
         struct Tiling { int tb, nt; };
         Tiling pick{tb, nt};
-        for (int large = 0; large < 2; large++) {
-            const Tiling T{tile_bits_for(prec, nrhs, large), large ? 512 : 256};
+        // 32 KiB tiles with 256 or 512 threads (4 or 2 rows per thread: the
+        // latter doubles the waves per CU), 64 KiB tiles with 512 threads
+        const Tiling cands[3] = {{tile_bits_for(prec, nrhs, 0), 256},
+                                 {tile_bits_for(prec, nrhs, 0), 512},
+                                 {tile_bits_for(prec, nrhs, 1), 512}};
+        for (const Tiling &T : cands) {
             if (T.tb > minlen) continue;
             if ((st = build_batch_tables(b, T.tb, T.nt, bits, coeff, term_off))) break;
             if ((st = launch_apply(b, y, x, 0))) break;   // warm-up
@@ -843,7 +852,7 @@ int rp_batch_create(rp_batch_t *out, int prec, int nprob, const int *len, const 
         (void)hipEventDestroy(e1);
         (void)hipFree(x);
         (void)hipFree(y);
-        if (!st && (pick.tb != b->tile_bits))
+        if (!st && (pick.tb != b->tile_bits || pick.nt != b->threads))
             st = build_batch_tables(b, pick.tb, pick.nt, bits, coeff, term_off);
         if (st) {
             batch_release(b);

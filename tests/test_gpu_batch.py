@@ -34,14 +34,17 @@ def tile_bits(fp, nrhs, large):
 
 @pytest.mark.parametrize("fp", [32, 64])
 @pytest.mark.parametrize("nrhs", [1, 2, 4])
-@pytest.mark.parametrize("large", [0, 1])
+@pytest.mark.parametrize("large", [0, 1, 2])
 def test_batch_apply_vs_oracle(gpu, orc, fp, nrhs, large):
-    tb = tile_bits(fp, nrhs, large)
-    rng = np.random.default_rng(fp * 100 + nrhs * 10 + tb)
+    """large 0: 32 KiB tiles, 256 threads; 2: the same tiles with 512
+    threads (RP_BATCH_WIDE); 1: 64 KiB tiles, 512 threads."""
+    tb = tile_bits(fp, nrhs, 1 if large == 1 else 0)
+    rng = np.random.default_rng(fp * 100 + nrhs * 10 + tb + large)
     lens = [14, 16, 15, 14] if tb > 12 else [12, 14, 13, 12]
     terms = problems(orc, rng, lens, fp)
-    b = gpu.Batch(lens, terms, nrhs=nrhs, kmax=4, fp=fp, tile_bits=tb)
+    b = gpu.Batch(lens, terms, nrhs=nrhs, kmax=4, fp=fp, tile_bits=tb, wide=(large == 2))
     assert b.info.tile_bits == tb
+    assert b.info.threads == (256 if large == 0 else 512)
     dt = np.complex64 if fp == 32 else np.complex128
     states = [np.stack([random_state(rng, L, dt) for _ in range(nrhs)], 1) for L in lens]
     b.load(states)
