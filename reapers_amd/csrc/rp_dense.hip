@@ -319,12 +319,17 @@ int rp_dense_eigh(int prec, uint64_t n, void *mat, void *w)
         ok = std::isfinite(nrm);
         return r;
     };
+    auto hip_ok = [&](hipError_t e, const char *what) -> int {
+        if (e == hipSuccess) return RP_OK;
+        return set_error(RP_ERR_RUNTIME, "rp_dense_eigh: %s failed: %s", what, hipGetErrorString(e));
+    };
     auto restore = [&]() -> int {
-        RP_CHECK_HIP(hipMemcpyAsync(mat, saved, mbytes, hipMemcpyDeviceToDevice, c.stream));
-        return RP_OK;
+        return hip_ok(hipMemcpyAsync(mat, saved, mbytes, hipMemcpyDeviceToDevice, c.stream), "copy");
     };
     do {
-        RP_CHECK_HIP(hipMemcpyAsync(saved, mat, mbytes, hipMemcpyDeviceToDevice, c.stream));
+        if ((rc = hip_ok(hipMemcpyAsync(saved, mat, mbytes, hipMemcpyDeviceToDevice, c.stream),
+                         "copy")))
+            break;
         rc = eigh_call(prec, h, true, n, mat, w, work, dinfo, &s);
         if (rc) break;
         bool ok = false;
@@ -349,14 +354,16 @@ int rp_dense_eigh(int prec, uint64_t n, void *mat, void *w)
         else
             hipLaunchKernelGGL(k_mixing_unitary<double>, dim3(grid_of(n * n)), dim3(kDenseThreads), 0,
                                c.stream, (double2 *)U, n, (uint64_t)0x5eedf00dull);
-        RP_CHECK_LAUNCH();
+        if ((rc = hip_ok(hipGetLastError(), "launch"))) break;
         if ((rc = rp_dense_gemm(prec, n, n, n, 0, 0, 1, 0, saved, n, U, n, 0, 0, tmp, n))) break;
         if ((rc = rp_dense_gemm(prec, n, n, n, 1, 0, 1, 0, U, n, tmp, n, 0, 0, mat, n))) break;
         rc = eigh_call(prec, h, true, n, mat, w, work, dinfo, &s);
         if (rc) break;
         if ((rc = finite(ok))) break;
         if (ok) {
-            RP_CHECK_HIP(hipMemcpyAsync(tmp, mat, mbytes, hipMemcpyDeviceToDevice, c.stream));
+            if ((rc = hip_ok(hipMemcpyAsync(tmp, mat, mbytes, hipMemcpyDeviceToDevice, c.stream),
+                             "copy")))
+                break;
             rc = rp_dense_gemm(prec, n, n, n, 0, 0, 1, 0, U, n, tmp, n, 0, 0, mat, n);
             break;
         }
