@@ -149,7 +149,7 @@ int rp_terms_info(rp_terms_t h, rp_terms_info_t *info);
 /* Rebuild the layout for a given tiling of the tiled kernel (no counterpart
  * in the reference): tile bits / threads / pipeline depth must be one of the
  * instantiated combinations -- (8,128,2), (10,128,2), (12,512,2) for both
- * precisions; (12,512,3), (12,512,4), (13,1024,2) for RP_C64; (11,256,4) for
+ * precisions; (12,512,3), (12,512,4), (12,1024,2), (13,1024,2) for RP_C64; (11,256,4) for
  * RP_C128 -- or tile bits 0 with 256 threads: the one-thread-per-amplitude
  * kernel, its depth the number of term slices (1, 8, 32, 128) that
  * separate workgroups sum before a second pass adds them in slice order.

@@ -156,7 +156,8 @@ constexpr int apply_waves_per_eu()
 {
     constexpr int lds = D * (1 << TB) * 2 * (int)sizeof(R);
     constexpr int wg_by_lds = (160 * 1024) / lds;
-    constexpr int wg_by_waves = 1024 / NT;
+    // 1024-thread workgroups: two per CU (32 waves) when their LDS fits
+    constexpr int wg_by_waves = (NT == 1024 ? 2048 : 1024) / NT;
     constexpr int wg = wg_by_lds < wg_by_waves ? wg_by_lds : wg_by_waves;
     constexpr int w = (wg < 1 ? 1 : wg) * (NT / 64) / 4;
     return w < 1 ? 1 : w;
@@ -469,7 +470,7 @@ constexpr TilingDesc kTilings[] = {
     {0, kSmallThreads, 1, 0}, {0, kSmallThreads, 8, 0}, {0, kSmallThreads, 32, 0},
     {0, kSmallThreads, 128, 0},
     {8, 128, 2, 0}, {10, 128, 2, 0}, {11, 256, 4, 16}, {12, 512, 2, 0},
-    {12, 512, 3, 8}, {12, 512, 4, 8}, {13, 1024, 2, 8}};
+    {12, 512, 3, 8}, {12, 512, 4, 8}, {12, 1024, 2, 8}, {13, 1024, 2, 8}};
 
 static bool tiling_instantiated(int prec, int tb, int nt, int d)
 {
@@ -635,6 +636,9 @@ static int apply_impl(rp_terms *h, void *res, const void *vec, int accumulate)
         break;
     case tiling_key(12, 512, 4):
         if constexpr (sizeof(R) == 4) return launch_tiled<R, A, 12, 512, 4>(h, res, vec, accumulate);
+        break;
+    case tiling_key(12, 1024, 2):
+        if constexpr (sizeof(R) == 4) return launch_tiled<R, A, 12, 1024>(h, res, vec, accumulate);
         break;
     case tiling_key(13, 1024, 2):
         if constexpr (sizeof(R) == 4) return launch_tiled<R, A, 13, 1024>(h, res, vec, accumulate);

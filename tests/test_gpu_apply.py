@@ -108,13 +108,13 @@ def test_autotuned_upload_matches_oracle(gpu, orc, fp, L):
     assert (info.tile_bits, info.threads, info.depth) in [
         (0, 256, 1), (0, 256, 8), (0, 256, 32), (0, 256, 128),
         (8, 128, 2), (10, 128, 2), (11, 256, 4), (12, 512, 2), (12, 512, 3), (12, 512, 4),
-        (13, 1024, 2)]
+        (12, 1024, 2), (13, 1024, 2)]
     assert np.abs(got - ref).max() < (1e-5 if fp == 32 else 1e-12) * np.abs(ref).max()
 
 
 SMALL = [(0, 256, 1), (0, 256, 8), (0, 256, 32), (0, 256, 128)]   # terms split in slices
 TILINGS = {32: SMALL + [(8, 128, 2), (10, 128, 2), (12, 512, 2), (12, 512, 3), (12, 512, 4),
-                        (13, 1024, 2)],
+                        (12, 1024, 2), (13, 1024, 2)],
            64: SMALL + [(8, 128, 2), (10, 128, 2), (11, 256, 4), (12, 512, 2)]}
 
 
@@ -156,7 +156,7 @@ def test_every_tiling_matches_oracle(gpu, orc, fp):
         vin.free(); vout.free(); t.free()
 
 
-@pytest.mark.parametrize("tiling", [(12, 512), (13, 1024), (10, 128)])
+@pytest.mark.parametrize("tiling", [(12, 512), (12, 1024), (13, 1024), (10, 128)])
 def test_group_accumulation_matches_oracle(gpu, orc, tiling):
     """RP_TERMS_ACC_GROUP (FP32 sums within a tile group, FP64 across
     groups) against the FP64 oracle: as accurate as the FP32 input allows."""
