@@ -6,6 +6,7 @@
                        (g++, links libreapers_hip.so)
 - reapers_service    : the JSON request/response program (g++, links
                        libreapers_hip.so)
+- examples/*         : example programs of the C++ API (g++)
 
 Run ``python -m reapers_amd.build`` or call :func:`build_all`.
 """
@@ -109,6 +110,28 @@ def build_service(force=False):
     return exe
 
 
+def build_examples(force=False):
+    """The example programs under examples/ (g++, links libreapers_hip.so)."""
+    deps = (_headers(os.path.join(HERE, "include"))
+            + [os.path.join(ROOT, "include", "reapers_hip.h"),
+               os.path.join(HERE, "libreapers_hip.so")])
+    edir = os.path.join(HERE, "examples")
+    out = []
+    for src in sorted(os.listdir(edir)):
+        if not src.endswith(".cc"):
+            continue
+        s = os.path.join(edir, src)
+        exe = os.path.join(edir, src[:-3])
+        if force or _stale(exe, [s] + deps):
+            tmp = exe + ".tmp"
+            _run([CXX] + HOST_FLAGS + ["-I" + os.path.join(HERE, "include", "reapers"),
+                                       "-o", tmp, s, "-L" + HERE, "-lreapers_hip",
+                                       "-Wl,-rpath,$ORIGIN/.."])
+            os.replace(tmp, exe)
+        out.append(exe)
+    return out
+
+
 def build_oracle(force=False):
     odir = os.path.join(ROOT, "oracle")
     if force:
@@ -122,6 +145,7 @@ def build_all(force=False):
     build_hip(force)
     build_host(force)
     build_service(force)
+    build_examples(force)
     build_oracle(force)
 
 
