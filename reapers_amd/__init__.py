@@ -199,6 +199,7 @@ def host():
             "rph_ham_dense": ([_vp, _c.c_int, _dbl, _dbl, _vp, _vp], _c.c_int),
             "rph_ham_matexp_entries": ([_vp, _c.POINTER(_i64)], _c.c_int),
             "rph_state_set": ([_vp, _u64, _dbl, _dbl], _c.c_int),
+            "rph_set_dense_apply": ([_c.c_int], _c.c_int),
             "rph_cache_budget": ([_i64, _c.POINTER(_i64), _c.POINTER(_i64)], _c.c_int),
             "rph_ham_to_string": ([_c.c_int, _vp, _vp, _i64, _c.c_char_p, _i64,
                                    _c.POINTER(_i64)], _c.c_int),
@@ -338,6 +339,12 @@ def set_upload_flags(flags):
 
 def init(device=0):
     check(hip().rp_init(device))
+
+
+def set_dense_apply(on):
+    """Apply through the dense matrix when the term count reaches 2^len (the
+    reference's small-problem switch); returns the previous setting."""
+    return bool(host().rph_set_dense_apply(int(bool(on))))
 
 
 def matexp_cache_budget(max_bytes=None):

@@ -1179,3 +1179,16 @@ int rph_cache_budget(int64_t set_bytes, int64_t *max_bytes, int64_t *used_bytes)
 }
 
 }  // extern "C"
+
+extern "C" {
+
+// GPUImpl::dense_apply(): apply through the dense matrix when the term count
+// reaches 2^len (up to 15 sites); returns the previous setting
+int rph_set_dense_apply(int on)
+{
+    int prev = GPUImpl::dense_apply() ? 1 : 0;
+    GPUImpl::dense_apply() = on != 0;
+    return prev;
+}
+
+}  // extern "C"

