@@ -231,7 +231,11 @@ int rp_dense_eye(int prec, uint64_t n, void *mat);
 #define RP_KERNEL_GS    1           /* rp_gs_pass launches    */
 #define RP_KERNEL_OTHER 2           /* every other launch     */
 /* when enabled, each launch is bracketed by events and its duration added to
- * a per-class total at the next synchronization */
+ * a per-class total at the next synchronization; on = RP_PROF_CLASSES(mask)
+ * brackets only the classes whose bit (1 << class) is set, any other
+ * nonzero value all of them */
+#define RP_PROF_SELECT 0x100
+#define RP_PROF_CLASSES(mask) (RP_PROF_SELECT | ((mask) & 0xff))
 int rp_prof_enable(int on);
 int rp_prof_reset(void);
 int rp_prof_query(int kernel_class, uint64_t *launches, double *total_ms);

@@ -38,6 +38,7 @@ struct Context {
     size_t pinned_doubles = 0;
     // profiling
     bool prof = false;
+    int prof_mask = 7;                // kernel classes bracketed (bit = class)
     uint64_t prof_launches[3] = {0, 0, 0};
     double prof_ms[3] = {0, 0, 0};
     std::vector<ProfPending> pending;
@@ -93,8 +94,9 @@ inline bool valid_prec(int prec) { return prec == RP_C64 || prec == RP_C128; }
     do {                                                                          \
         ::rp::Context &c_ = ::rp::ctx();                                          \
         hipEvent_t s_ = nullptr, t_ = nullptr;                                    \
-        if (c_.prof) ::rp::prof_begin((cls), &s_, &t_);                           \
+        const bool p_ = c_.prof && ((c_.prof_mask >> (cls)) & 1);                 \
+        if (p_) ::rp::prof_begin((cls), &s_, &t_);                                \
         hipLaunchKernelGGL(kernel, (grid), (block), (shmem), c_.stream, __VA_ARGS__); \
         RP_CHECK_LAUNCH();                                                        \
-        if (c_.prof) ::rp::prof_end((cls), s_, t_);                               \
+        if (p_) ::rp::prof_end((cls), s_, t_);                                    \
     } while (0)

@@ -216,6 +216,8 @@ int rp_prof_enable(int on)
     int st = check_init();
     if (st) return st;
     ctx().prof = on != 0;
+    // RP_PROF_CLASSES(mask) selects kernel classes; any other nonzero value all
+    ctx().prof_mask = (on & RP_PROF_SELECT) ? (on & 0xff) : 0xff;
     return RP_OK;
 }
 
