@@ -193,7 +193,7 @@ def host():
             "rph_syk_block_npoint": ([_c.c_int, _c.c_int, _c.c_int, _dbl, _dbl, _c.c_uint32,
                                       _c.c_int, _c.c_int, _vp, _u64, _dbl, _dbl, _c.c_int,
                                       _c.c_int, _dbl, _vp, _c.POINTER(_dbl),
-                                      _c.POINTER(EvolveStats), _vp], _c.c_int),
+                                      _c.POINTER(EvolveStats), _vp, _c.c_int], _c.c_int),
             "rph_state_evolve_ed": ([_vp, _vp, _dbl, _dbl], _c.c_int),
             "rph_state_ground_ed": ([_vp, _vp, _c.POINTER(_dbl)], _c.c_int),
             "rph_ham_dense": ([_vp, _c.c_int, _dbl, _dbl, _vp, _vp], _c.c_int),
@@ -609,13 +609,15 @@ class BlockState:
 
 def syk_block_npoint(N, kind="otoc", sp=0.0, J=1.0, seed=0, regularize=False, otoc_idx=-1,
                      init=None, state_seed=1, beta=0.0, tmax=1.0, nsteps=10, krylov_dim=5,
-                     tol=0.0, fp=32, schedule=None):
+                     tol=0.0, fp=32, schedule=None, promote=False):
     """One disorder realization of the parity-block SYK OTOC or Green's
     function: returns (t, v, e0, stats) with v_k / v_0 at t_k = k tmax/nsteps
     and e0 the ground-state energy the Hamiltonian was shifted by.  kind
     'otoc-trace' / 'green-trace' use the trace definitions by exact
     diagonalization (no state; e0 = 0).  A Schedule chooses each restart's
-    Krylov dimension (krylov_dim stays the fallback and the Lanczos size)."""
+    Krylov dimension (krylov_dim stays the fallback and the Lanczos size).
+    promote (fp=64) generates H and the random state in FP32 and converts
+    them: the FP64 counterpart of an FP32 run on the same couplings."""
     k = {"otoc": 0, "green": 1, "otoc-trace": 2, "green-trace": 3}[kind]
     out = np.zeros(2 * (nsteps + 1))
     e0 = _dbl()
@@ -627,7 +629,8 @@ def syk_block_npoint(N, kind="otoc", sp=0.0, J=1.0, seed=0, regularize=False, ot
     hcheck(host().rph_syk_block_npoint(k, fp, N, sp, J, seed, int(regularize), otoc_idx,
                                        None if a is None else _ptr(a), state_seed, beta, tmax,
                                        nsteps, krylov_dim, tol, _ptr(out), _c.byref(e0),
-                                       _c.byref(st), None if schedule is None else schedule._h))
+                                       _c.byref(st), None if schedule is None else schedule._h,
+                                       int(promote)))
     t = np.arange(nsteps + 1) * (tmax / nsteps if nsteps else 0.0)
     return t, out[0::2] + 1j * out[1::2], e0.value, st
 

@@ -734,18 +734,37 @@ void block_apply(BlkSt *s, Ham *LL, Ham *LR, Ham *RL, Ham *RR)
     }
 }
 
+// A view of the term list of one precision converted to another
+template<RealScalar To, RealScalar From>
+SubspaceView<SumOps<To>> promote_view(const SubspaceView<SumOps<From>> &v)
+{
+    return SubspaceView<SumOps<To>>(v.spin_chain_length(), SumOps<To>(v));
+}
+
 template<RealScalar Fp>
 void syk_block_npoint(int kind, int N, double sp, double J, uint32_t seed, int regularize,
                       int otoc_idx, const void *init, uint64_t state_seed, double beta,
                       double tmax, int nsteps, int krylov_dim, double tol, double *out,
-                      double *e0, KrylovStats &st, void *sched)
+                      double *e0, KrylovStats &st, void *sched, int promote)
 {
-    SYKBlockForm<Fp, GPUImpl> syk(N, (Fp)sp, (Fp)J);
+    // the Hamiltonian and the Majorana block in Fp; with promote (Fp =
+    // double) generated in FP32 and converted, the reference's FP32/FP64
+    // cross-check (examples/syk.cc:440-474)
+    using View = SubspaceView<SumOps<Fp>>;
+    BlockDiag<View> ham;
+    View op;
     std::mt19937 rg(seed);
-    auto ham = syk.gen_ham(rg, regularize != 0);
-    const auto &ops = syk.fermion_ops();
-    auto op = ops[N - 2].LR;
-    if (otoc_idx >= 0 && otoc_idx <= N - 2) op = ops[otoc_idx].LR;
+    const int oi = (otoc_idx >= 0 && otoc_idx <= N - 2) ? otoc_idx : N - 2;
+    if (promote && std::is_same_v<Fp, double>) {
+        SYKBlockForm<float, GPUImpl> syk(N, (float)sp, (float)J);
+        auto h = syk.gen_ham(rg, regularize != 0);
+        ham = BlockDiag<View>(promote_view<Fp>(h.LL), promote_view<Fp>(h.RR));
+        op = promote_view<Fp>(syk.fermion_ops()[oi].LR);
+    } else {
+        SYKBlockForm<Fp, GPUImpl> syk(N, (Fp)sp, (Fp)J);
+        ham = syk.gen_ham(rg, regularize != 0);
+        op = syk.fermion_ops()[oi].LR;
+    }
     if (kind >= 2) {
         auto v = npoint_block_trace<Fp>(kind == 2 ? NPoint::Otoc : NPoint::Green, ham, op, tmax,
                                         nsteps, beta);
@@ -757,8 +776,15 @@ void syk_block_npoint(int kind, int N, double sp, double J, uint32_t seed, int r
         return;
     }
     State<Fp, GPUImpl> s0(N / 2 - 1);
-    if (init) s0.load((const complex<Fp> *)init);
-    else s0.random_state(state_seed);
+    if (init) {
+        s0.load((const complex<Fp> *)init);
+    } else if (promote && std::is_same_v<Fp, double>) {
+        State<float, GPUImpl> s32(N / 2 - 1);
+        s32.random_state(state_seed);
+        s0 = State<Fp, GPUImpl>(s32);
+    } else {
+        s0.random_state(state_seed);
+    }
     KrylovOptions opt;
     opt.krydim = krylov_dim;
     opt.tol = tol;
@@ -990,11 +1016,14 @@ int rph_block_state_ground(void *sp, void *LL, void *RR, int krydim, double eps,
 // out receives 2 (nsteps + 1) doubles (Re, Im of v_k / v_0), e0 the
 // ground-state energy the Hamiltonian was shifted by; sched (nullable) is a
 // schedule handle choosing each restart's Krylov dimension (krylov_dim is
-// then the fallback; the Lanczos E0 always uses krylov_dim).
+// then the fallback; the Lanczos E0 always uses krylov_dim); promote (fp 64
+// only) generates the Hamiltonian and the random initial state in FP32 and
+// converts them, so that an FP32 run has an FP64 counterpart on the same
+// couplings.
 int rph_syk_block_npoint(int kind, int fp, int N, double sp, double J, uint32_t seed,
                          int regularize, int otoc_idx, const void *init, uint64_t state_seed,
                          double beta, double tmax, int nsteps, int krylov_dim, double tol,
-                         double *out, double *e0, rph_stats *stats, void *sched)
+                         double *out, double *e0, rph_stats *stats, void *sched, int promote)
 {
     return guarded([&] {
         if (kind < 0 || kind > 3) ThrowException(InvalidArgument, "kind", "must be 0..3");
@@ -1003,10 +1032,12 @@ int rph_syk_block_npoint(int kind, int fp, int N, double sp, double J, uint32_t 
         auto t0 = std::chrono::steady_clock::now();
         if (fp == 32)
             syk_block_npoint<float>(kind, N, sp, J, seed, regularize, otoc_idx, init, state_seed,
-                                    beta, tmax, nsteps, krylov_dim, tol, out, e0, st, sched);
+                                    beta, tmax, nsteps, krylov_dim, tol, out, e0, st, sched,
+                                    0);
         else
             syk_block_npoint<double>(kind, N, sp, J, seed, regularize, otoc_idx, init, state_seed,
-                                     beta, tmax, nsteps, krylov_dim, tol, out, e0, st, sched);
+                                     beta, tmax, nsteps, krylov_dim, tol, out, e0, st, sched,
+                                     promote);
         GPUImpl::synchronize();
         double wall = std::chrono::duration<double, std::milli>(
                           std::chrono::steady_clock::now() - t0).count();

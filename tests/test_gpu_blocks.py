@@ -240,3 +240,20 @@ def test_syk_block_npoint_with_proposer_vs_ed(gpu, orc):
     print("block OTOC with the proposer: %.2e, matvecs %d" % (np.abs(v - ref).max(), st.matvecs),
           flush=True)
     assert np.abs(v - ref).max() < 1e-10
+
+
+def test_syk_block_npoint_promoted_fp64_cross_check(gpu):
+    """promote=True runs FP64 on the FP32 run's couplings and initial state
+    (the reference's cross-check, examples/syk.cc:440-474): the FP32 OTOC
+    agrees with it to FP32 accuracy, while a native FP64 run (its own
+    couplings) does not."""
+    N, seed, beta = 24, 31, 1.0
+    kw = dict(seed=seed, state_seed=77, beta=beta, tmax=4.0, nsteps=4, krylov_dim=16)
+    _, v32, _, _ = gpu.syk_block_npoint(N, "otoc", tol=1e-7, fp=32, **kw)
+    _, v64p, _, _ = gpu.syk_block_npoint(N, "otoc", tol=1e-12, fp=64, promote=True, **kw)
+    _, v64n, _, _ = gpu.syk_block_npoint(N, "otoc", tol=1e-12, fp=64, **kw)
+    err = np.abs(v32 - v64p).max()
+    print("FP32 vs promoted FP64: %.2e; vs native FP64: %.2e" % (err, np.abs(v32 - v64n).max()),
+          flush=True)
+    assert err < 1e-5
+    assert np.abs(v32 - v64n).max() > 1e-3
