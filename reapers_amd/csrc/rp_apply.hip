@@ -175,7 +175,10 @@ __device__ inline void tile_dma(const T *__restrict__ in, uint64_t src, T *dst)
     static_assert(PER_THREAD >= 1 && TILE_BYTES % (16 * NT) == 0, "tile too small for DMA");
     const char *g = (const char *)(in + src);
     char *l = (char *)dst;
-    const uint32_t tid = threadIdx.x, wave_base = tid & ~63u;
+    // wave-uniform: kept in a scalar register, so that the LDS address
+    // (M0) of every DMA is scalar arithmetic
+    const uint32_t tid = threadIdx.x,
+                   wave_base = __builtin_amdgcn_readfirstlane(tid & ~63u);
 #pragma unroll
     for (int c = 0; c < PER_THREAD; c++) {
         const uint32_t chunk = c * NT;
@@ -201,14 +204,29 @@ __device__ __forceinline__ P sub_im(const SubRec<A> &S)
     else return (P)S.ci;
 }
 
+// Sign bit (bit 31) of (-1)^popcount(v), and a coefficient with that sign
+// applied by flipping its sign bit
+__device__ __forceinline__ uint32_t parity_sign(uint32_t v) { return (uint32_t)__popc(v) << 31; }
+__device__ __forceinline__ float flip_sign(float c, uint32_t sb)
+{
+    return __uint_as_float(__float_as_uint(c) ^ sb);
+}
+__device__ __forceinline__ double flip_sign(double c, uint32_t sb)
+{
+    return __longlong_as_double(__double_as_longlong(c) ^ ((long long)sb << 32));
+}
+
 // Apply every sub of group G from the staged input tile `buf` (LDS) to the
-// RP accumulated rows of this thread.
+// RP accumulated rows of this thread.  Amplitude t0 + r NT + tid of the
+// output tile reads the input at (t0 + r NT + tid) ^ x: lane tid ^ (x mod NT)
+// of tile row r ^ (x / NT mod RP); lanebits = t0 | tid (the index bits
+// outside the row bits) gives the lane's share of the phase sign.
 template <typename R, typename A, int TB, int NT, typename P = A>
 __device__ __forceinline__ void apply_group(const GroupRuns &G, const typename CxA<R>::T *buf,
                                             const SubRec<A> *__restrict__ subs,
-                                            const TermRec<A> *__restrict__ terms, uint32_t t0,
-                                            uint32_t tid, P (&accr)[(1 << TB) / NT],
-                                            P (&acci)[(1 << TB) / NT])
+                                            const TermRec<A> *__restrict__ terms,
+                                            uint32_t lanebits, uint32_t tid,
+                                            P (&accr)[(1 << TB) / NT], P (&acci)[(1 << TB) / NT])
 {
     using T = typename CxA<R>::T;
     constexpr int RP = (1 << TB) / NT;
@@ -218,13 +236,13 @@ __device__ __forceinline__ void apply_group(const GroupRuns &G, const typename C
     const SubRec<A> S = subs[s];                                               \
     const T *row = buf + (tid ^ (S.xl & (NT - 1)));                            \
     const uint32_t xr = S.xl >> LB;                                            \
-    const bool lneg = parity32((t0 | tid) & S.z);                              \
+    const uint32_t lsb = parity_sign(lanebits & S.z);                          \
     const uint32_t zr = (S.z >> LB) & (RP - 1);                                \
     (void)zr;
     // real c, no phase bit among the row bits
     for (uint32_t s = G.b[kRunReal0]; s < G.b[kRunReal0 + 1]; s++) {
         RP_SUB_PROLOGUE
-        const P c0 = sub_re<P>(S), c = lneg ? -c0 : c0;
+        const P c = flip_sign(sub_re<P>(S), lsb);
 #pragma unroll
         for (int r = 0; r < RP; r++) {
             const T v = row[(r ^ xr) * NT];
@@ -234,11 +252,11 @@ __device__ __forceinline__ void apply_group(const GroupRuns &G, const typename C
     }
     for (uint32_t s = G.b[kRunReal1]; s < G.b[kRunReal1 + 1]; s++) {
         RP_SUB_PROLOGUE
-        const P c0 = sub_re<P>(S), c = lneg ? -c0 : c0;
+        const P c = flip_sign(sub_re<P>(S), lsb);
 #pragma unroll
         for (int r = 0; r < RP; r++) {
             const T v = row[(r ^ xr) * NT];
-            const P cc = parity32(r & zr) ? -c : c;
+            const P cc = flip_sign(c, parity_sign(r & zr));
             accr[r] += cc * (P)v.x;
             acci[r] += cc * (P)v.y;
         }
@@ -246,7 +264,7 @@ __device__ __forceinline__ void apply_group(const GroupRuns &G, const typename C
     // imaginary c = i ci: (i ci)(vr + i vi) = -ci vi + i ci vr
     for (uint32_t s = G.b[kRunImag0]; s < G.b[kRunImag0 + 1]; s++) {
         RP_SUB_PROLOGUE
-        const P c0 = sub_im<P>(S), c = lneg ? -c0 : c0;
+        const P c = flip_sign(sub_im<P>(S), lsb);
 #pragma unroll
         for (int r = 0; r < RP; r++) {
             const T v = row[(r ^ xr) * NT];
@@ -256,28 +274,28 @@ __device__ __forceinline__ void apply_group(const GroupRuns &G, const typename C
     }
     for (uint32_t s = G.b[kRunImag1]; s < G.b[kRunImag1 + 1]; s++) {
         RP_SUB_PROLOGUE
-        const P c0 = sub_im<P>(S), c = lneg ? -c0 : c0;
+        const P c = flip_sign(sub_im<P>(S), lsb);
 #pragma unroll
         for (int r = 0; r < RP; r++) {
             const T v = row[(r ^ xr) * NT];
-            const P cc = parity32(r & zr) ? -c : c;
+            const P cc = flip_sign(c, parity_sign(r & zr));
             accr[r] -= cc * (P)v.y;
             acci[r] += cc * (P)v.x;
         }
     }
     for (uint32_t s = G.b[kRunCplx]; s < G.b[kRunCplx + 1]; s++) {
         RP_SUB_PROLOGUE
-        const P r0 = sub_re<P>(S), i0 = sub_im<P>(S);
-        const P cr = lneg ? -r0 : r0, ci = lneg ? -i0 : i0;
+        const P cr = flip_sign(sub_re<P>(S), lsb), ci = flip_sign(sub_im<P>(S), lsb);
 #pragma unroll
         for (int r = 0; r < RP; r++) {
             const T v = row[(r ^ xr) * NT];
-            const bool n = parity32(r & zr);
-            const P a = n ? -cr : cr, c = n ? -ci : ci;
+            const uint32_t rsb = parity_sign(r & zr);
+            const P a = flip_sign(cr, rsb), c = flip_sign(ci, rsb);
             accr[r] += a * (P)v.x - c * (P)v.y;
             acci[r] += a * (P)v.y + c * (P)v.x;
         }
     }
+#undef RP_SUB_PROLOGUE
     // several terms share x: d(m) = sum_t c_t (-1)^popcount(m & z_t),
     // formed for four rows at a time to bound register use
     for (uint32_t s = G.b[kRunMulti]; s < G.b[kRunMulti + 1]; s++) {
@@ -292,15 +310,15 @@ __device__ __forceinline__ void apply_group(const GroupRuns &G, const typename C
             for (int q = 0; q < RC; q++) { dr[q] = 0; di[q] = 0; }
             for (uint32_t t = S.term_begin; t < S.term_end; t++) {
                 const TermRec<A> C = terms[t];
-                const bool lneg = parity32((t0 | tid) & C.z);
-                const P cr = (P)(lneg ? -C.cr : C.cr);
-                const P ci = (P)(lneg ? -C.ci : C.ci);
+                const uint32_t lsb = parity_sign(lanebits & C.z);
+                const P cr = (P)flip_sign(C.cr, lsb);
+                const P ci = (P)flip_sign(C.ci, lsb);
                 const uint32_t zr = (C.z >> LB) & (RP - 1);
 #pragma unroll
                 for (int q = 0; q < RC; q++) {
-                    const bool n = parity32((r0 + q) & zr);
-                    dr[q] += n ? -cr : cr;
-                    di[q] += n ? -ci : ci;
+                    const uint32_t rsb = parity_sign((r0 + q) & zr);
+                    dr[q] += flip_sign(cr, rsb);
+                    di[q] += flip_sign(ci, rsb);
                 }
             }
 #pragma unroll
@@ -311,7 +329,6 @@ __device__ __forceinline__ void apply_group(const GroupRuns &G, const typename C
             }
         }
     }
-#undef RP_SUB_PROLOGUE
 }
 
 // s_waitcnt immediate that waits until at most N vector memory operations
@@ -369,9 +386,9 @@ __device__ __forceinline__ void pipe_phase(
     if constexpr (GA) {
         // the terms of a pair of groups summed in FP32 (gr, gi), the pair
         // sums added to the FP64 accumulators by the caller
-        apply_group<R, A, TB, NT, float>(groups[g], bufs[K], subs, terms, t0, tid, gr, gi);
+        apply_group<R, A, TB, NT, float>(groups[g], bufs[K], subs, terms, t0 | tid, tid, gr, gi);
     } else {
-        apply_group<R, A, TB, NT>(groups[g], bufs[K], subs, terms, t0, tid, accr, acci);
+        apply_group<R, A, TB, NT>(groups[g], bufs[K], subs, terms, t0 | tid, tid, accr, acci);
     }
 }
 
@@ -386,6 +403,9 @@ __device__ __forceinline__ void flush_partials(A (&accr)[RP], A (&acci)[RP], flo
         acci[r] += (A)gi[r];
         gr[r] = 0;
         gi[r] = 0;
+        // opaque zeros: the compiler would otherwise propagate the constant
+        // into the next round and zero the registers again on every path
+        asm volatile("" : "+v"(gr[r]), "+v"(gi[r]));
     }
 }
 
@@ -417,7 +437,13 @@ k_apply_tiled(const typename CxA<R>::T *__restrict__ in, typename CxA<R>::T *__r
     A accr[RP], acci[RP];
     float gr[RP], gi[RP];      // FP32 partial sums (group accumulation)
 #pragma unroll
-    for (int r = 0; r < RP; r++) { accr[r] = 0; acci[r] = 0; gr[r] = 0; gi[r] = 0; }
+    for (int r = 0; r < RP; r++) {
+        accr[r] = 0;
+        acci[r] = 0;
+        gr[r] = 0;
+        gi[r] = 0;
+        if constexpr (GA) asm volatile("" : "+v"(gr[r]), "+v"(gi[r]));
+    }
 
     T *const ring[4] = {tile0, tile1, tile2, tile3};
 #pragma unroll
