@@ -173,6 +173,8 @@ __device__ inline void tile_dma(const T *__restrict__ in, uint64_t src, T *dst)
     constexpr int TILE_BYTES = (1 << TB) * (int)sizeof(T);
     constexpr int PER_THREAD = TILE_BYTES / (16 * NT);
     static_assert(PER_THREAD >= 1 && TILE_BYTES % (16 * NT) == 0, "tile too small for DMA");
+    // uniform base plus a 32-bit lane offset: the scalar-base addressing
+    // form, no 64-bit address arithmetic per lane
     const char *g = (const char *)(in + src);
     char *l = (char *)dst;
     // wave-uniform: kept in a scalar register, so that the LDS address
@@ -183,7 +185,7 @@ __device__ inline void tile_dma(const T *__restrict__ in, uint64_t src, T *dst)
     for (int c = 0; c < PER_THREAD; c++) {
         const uint32_t chunk = c * NT;
         __builtin_amdgcn_global_load_lds(
-            (const void *)(g + (size_t)(chunk + tid) * 16),
+            (const void *)(g + (uint32_t)((chunk + tid) * 16)),
             (__attribute__((address_space(3))) void *)(l + (size_t)(chunk + wave_base) * 16),
             16, 0, 0);
     }
@@ -207,13 +209,34 @@ __device__ __forceinline__ P sub_im(const SubRec<A> &S)
 // Sign bit (bit 31) of (-1)^popcount(v), and a coefficient with that sign
 // applied by flipping its sign bit
 __device__ __forceinline__ uint32_t parity_sign(uint32_t v) { return (uint32_t)__popc(v) << 31; }
+// (adding bit 31 flips it like an xor: one shift-add instruction with the
+// shift of the sign bit)
 __device__ __forceinline__ float flip_sign(float c, uint32_t sb)
 {
-    return __uint_as_float(__float_as_uint(c) ^ sb);
+    return __uint_as_float(__float_as_uint(c) + sb);
 }
 __device__ __forceinline__ double flip_sign(double c, uint32_t sb)
 {
     return __longlong_as_double(__double_as_longlong(c) ^ ((long long)sb << 32));
+}
+// the lane-signed coefficient of a sub whose rows flip it again: kept in a
+// register, so that each row's flip is one xor with a scalar sign instead
+// of the whole popcount per row
+template <typename P> __device__ __forceinline__ P opaque(P c)
+{
+    asm volatile("" : "+v"(c));
+    return c;
+}
+
+// The LDS address of this lane's input element in tile row 0 for flip mask
+// xl: the lane offset is formed once per sub (one xor) and kept opaque, so
+// that each row's address is that offset plus a scalar row offset
+template <typename T, int NT>
+__device__ __forceinline__ const T *sub_lane_base(const T *buf, uint32_t lane8, uint32_t xl)
+{
+    uint32_t la = lane8 ^ ((xl & (NT - 1)) * (uint32_t)sizeof(T));
+    asm volatile("" : "+v"(la));
+    return (const T *)((const char *)buf + la);
 }
 
 // Apply every sub of group G from the staged input tile `buf` (LDS) to the
@@ -232,9 +255,11 @@ __device__ __forceinline__ void apply_group(const GroupRuns &G, const typename C
     constexpr int RP = (1 << TB) / NT;
     constexpr int LB = NT == 1024 ? 10 : NT == 512 ? 9 : NT == 256 ? 8 : 7;
 
+    // this lane's byte offset in a tile row
+    const uint32_t lane8 = tid * (uint32_t)sizeof(T);
 #define RP_SUB_PROLOGUE                                                            \
     const SubRec<A> S = subs[s];                                               \
-    const T *row = buf + (tid ^ (S.xl & (NT - 1)));                            \
+    const T *row = sub_lane_base<T, NT>(buf, lane8, S.xl);                     \
     const uint32_t xr = S.xl >> LB;                                            \
     const uint32_t lsb = parity_sign(lanebits & S.z);                          \
     const uint32_t zr = (S.z >> LB) & (RP - 1);                                \
@@ -252,7 +277,7 @@ __device__ __forceinline__ void apply_group(const GroupRuns &G, const typename C
     }
     for (uint32_t s = G.b[kRunReal1]; s < G.b[kRunReal1 + 1]; s++) {
         RP_SUB_PROLOGUE
-        const P c = flip_sign(sub_re<P>(S), lsb);
+        const P c = opaque(flip_sign(sub_re<P>(S), lsb));
 #pragma unroll
         for (int r = 0; r < RP; r++) {
             const T v = row[(r ^ xr) * NT];
@@ -274,7 +299,7 @@ __device__ __forceinline__ void apply_group(const GroupRuns &G, const typename C
     }
     for (uint32_t s = G.b[kRunImag1]; s < G.b[kRunImag1 + 1]; s++) {
         RP_SUB_PROLOGUE
-        const P c = flip_sign(sub_im<P>(S), lsb);
+        const P c = opaque(flip_sign(sub_im<P>(S), lsb));
 #pragma unroll
         for (int r = 0; r < RP; r++) {
             const T v = row[(r ^ xr) * NT];
@@ -285,7 +310,7 @@ __device__ __forceinline__ void apply_group(const GroupRuns &G, const typename C
     }
     for (uint32_t s = G.b[kRunCplx]; s < G.b[kRunCplx + 1]; s++) {
         RP_SUB_PROLOGUE
-        const P cr = flip_sign(sub_re<P>(S), lsb), ci = flip_sign(sub_im<P>(S), lsb);
+        const P cr = opaque(flip_sign(sub_re<P>(S), lsb)), ci = opaque(flip_sign(sub_im<P>(S), lsb));
 #pragma unroll
         for (int r = 0; r < RP; r++) {
             const T v = row[(r ^ xr) * NT];
@@ -300,7 +325,7 @@ __device__ __forceinline__ void apply_group(const GroupRuns &G, const typename C
     // formed for four rows at a time to bound register use
     for (uint32_t s = G.b[kRunMulti]; s < G.b[kRunMulti + 1]; s++) {
         const SubRec<A> S = subs[s];
-        const T *row = buf + (tid ^ (S.xl & (NT - 1)));
+        const T *row = sub_lane_base<T, NT>(buf, lane8, S.xl);
         const uint32_t xr = S.xl >> LB;
         constexpr int RC = RP < 4 ? RP : 4;
 #pragma unroll
