@@ -122,7 +122,7 @@ int rp_terms_upload(rp_terms_t *out, int prec, int len, const uint64_t *bits,
 /* RP_TERMS_AUTOTUNE times the instantiated apply tilings (tile bits and
  * workgroup size) on scratch vectors at upload and keeps the fastest */
 #define RP_TERMS_AUTOTUNE 2
-/* RP_TERMS_ACC_GROUP (complex<float> only): the terms of each pair of
+/* RP_TERMS_ACC_GROUP (complex<float> only): the terms of every four
  * consecutive tile groups are summed in FP32 and those sums in FP64 -- FP32
  * arithmetic in the inner loop, the running sum never rounded to FP32 (used
  * by the 2-stage pipelines; others accumulate in FP64) */

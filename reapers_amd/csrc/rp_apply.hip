@@ -409,7 +409,7 @@ __device__ __forceinline__ void pipe_phase(
     if (g + D - 1 < ngroups)
         tile_dma<T, TB, NT>(in, t0 ^ (groups[g + D - 1].xh << TB), bufs[(K + D - 1) % D]);
     if constexpr (GA) {
-        // the terms of a pair of groups summed in FP32 (gr, gi), the pair
+        // the terms of four consecutive groups summed in FP32 (gr, gi), those
         // sums added to the FP64 accumulators by the caller
         apply_group<R, A, TB, NT, float>(groups[g], bufs[K], subs, terms, t0 | tid, tid, gr, gi);
     } else {
@@ -490,8 +490,11 @@ k_apply_tiled(const typename CxA<R>::T *__restrict__ in, typename CxA<R>::T *__r
             pipe_phase<R, A, TB, NT, D, 3, GA>(g + 3, ngroups, in, groups, subs, terms, t0, tid, tile0,
                                            tile1, tile2, tile3, accr, acci, gr, gi);
         }
-        // FP32 partial sums of this round's D groups into the FP64 accumulators
-        if constexpr (GA) flush_partials(accr, acci, gr, gi);
+        // FP32 partial sums of every other round (four groups at D = 2) into
+        // the FP64 accumulators
+        if constexpr (GA) {
+            if ((g / D) & 1) flush_partials(accr, acci, gr, gi);
+        }
     }
     if constexpr (GA) flush_partials(accr, acci, gr, gi);
 #pragma unroll
