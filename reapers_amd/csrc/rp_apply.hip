@@ -585,7 +585,7 @@ static int upload_impl(rp_terms *h, const uint64_t *bits, const double *coeff, u
     Context &c = ctx();
     auto up = [&](void **dst, const void *src, size_t bytes) -> int {
         if (!bytes) return RP_OK;
-        hipError_t e = hipMalloc(dst, bytes);
+        hipError_t e = dev_malloc((void **)dst, bytes);
         if (e != hipSuccess) {
             (void)hipGetLastError();
             return set_error(RP_ERR_OUT_OF_MEMORY, "Failed to allocate %zu bytes on GPU device.",
@@ -601,7 +601,7 @@ static int upload_impl(rp_terms *h, const uint64_t *bits, const double *coeff, u
             if (h->d_split) (void)hipFree(h->d_split);
             h->d_split = nullptr;
             h->split_bytes = 0;
-            if (hipMalloc(&h->d_split, need) != hipSuccess) {
+            if (dev_malloc((void **)&h->d_split, need) != hipSuccess) {
                 (void)hipGetLastError();
                 h->d_split = nullptr;
                 return set_error(RP_ERR_OUT_OF_MEMORY, "Failed to allocate %zu bytes on GPU device.",
@@ -744,7 +744,7 @@ static int autotune(rp_terms *h)
     Context &c = ctx();
     const size_t bytes = (size_t)elem_bytes(h->prec) << h->len;
     void *x = nullptr, *y = nullptr;
-    if (hipMalloc(&x, bytes) != hipSuccess || hipMalloc(&y, bytes) != hipSuccess) {
+    if (dev_malloc(&x, bytes) != hipSuccess || dev_malloc(&y, bytes) != hipSuccess) {
         (void)hipGetLastError();
         if (x) (void)hipFree(x);
         return set_error(RP_ERR_OUT_OF_MEMORY, "rp_terms_upload_ex: no memory for autotuning");

@@ -563,7 +563,7 @@ k_batch_lincomb(const void *const *__restrict__ Vt, int k, typename CxA<R>::T *_
 
 int batch_alloc(void **p, size_t bytes)
 {
-    hipError_t e = hipMalloc(p, bytes ? bytes : 16);
+    hipError_t e = dev_malloc((void **)p, bytes ? bytes : 16);
     if (e != hipSuccess) {
         (void)hipGetLastError();
         return set_error(RP_ERR_OUT_OF_MEMORY, "Failed to allocate %zu bytes on GPU device.",

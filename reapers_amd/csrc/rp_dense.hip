@@ -227,7 +227,7 @@ int rp_dense_build(int prec, int len, const uint64_t *bits, const double *coeff,
     uint64_t *dx = nullptr;
     void *dbuf = nullptr;
     size_t tbytes = nterms * (2 * sizeof(uint64_t) + 2 * sizeof(double));
-    RP_CHECK_HIP(hipMalloc(&dbuf, tbytes));
+    RP_CHECK_HIP(dev_malloc((void **)&dbuf, tbytes));
     dx = (uint64_t *)dbuf;
     uint64_t *dz = dx + nterms;
     double *dc = (double *)(dz + nterms);
@@ -305,8 +305,8 @@ int rp_dense_eigh(int prec, uint64_t n, void *mat, void *w)
     size_t rbytes = prec == RP_C64 ? sizeof(float) : sizeof(double);
     size_t mbytes = n * n * elem_bytes(prec);
     void *work = nullptr, *saved = nullptr, *U = nullptr, *tmp = nullptr;
-    RP_CHECK_HIP(hipMalloc(&work, n * rbytes + sizeof(rocblas_int)));
-    if (hipMalloc(&saved, mbytes) != hipSuccess) {
+    RP_CHECK_HIP(dev_malloc((void **)&work, n * rbytes + sizeof(rocblas_int)));
+    if (dev_malloc((void **)&saved, mbytes) != hipSuccess) {
         (void)hipFree(work);
         return set_error(RP_ERR_OUT_OF_MEMORY, "rp_dense_eigh: no memory for the matrix copy");
     }
@@ -337,7 +337,8 @@ int rp_dense_eigh(int prec, uint64_t n, void *mat, void *w)
         bool pow2 = (n & (n - 1)) == 0;
         // rotated (n a power of two): tmp = A U, mat = U^H tmp, heevd, then
         // V = U V' into mat
-        if (pow2 && (hipMalloc(&U, mbytes) != hipSuccess || hipMalloc(&tmp, mbytes) != hipSuccess)) {
+        if (pow2 && (dev_malloc((void **)&U, mbytes) != hipSuccess ||
+                     dev_malloc((void **)&tmp, mbytes) != hipSuccess)) {
             rc = set_error(RP_ERR_OUT_OF_MEMORY, "rp_dense_eigh: no memory for the rotation");
             break;
         }

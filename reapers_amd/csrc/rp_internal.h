@@ -49,6 +49,13 @@ Context &ctx();
 
 // record the last error of this thread and return the status code
 int set_error(int code, const char *fmt, ...);
+
+// Device allocation for every buffer of the library: hipMalloc, and when
+// that runs out of memory, the blocks kept by rp_free are returned to HIP
+// and the allocation is tried once more.
+hipError_t dev_malloc(void **ptr, size_t bytes);
+// bytes held in freed blocks kept for reuse
+size_t pooled_bytes();
 int check_init();
 
 // launch bracketing for the optional per-class timing

@@ -5,6 +5,9 @@
 #include "rp_internal.h"
 
 #include <cstdarg>
+#include <map>
+#include <mutex>
+#include <unordered_map>
 #include <cstdio>
 #include <cstring>
 
@@ -109,13 +112,21 @@ int rp_init(int device)
     // partials for up to 64 complex FP64 values and one norm per reduction
     // block, plus room for the reduced results
     c.work_bytes = (size_t)kRedBlocks * (2 * 64 + 1) * sizeof(double) + 64 * 1024;
-    RP_CHECK_HIP(hipMalloc(&c.work, c.work_bytes));
+    RP_CHECK_HIP(dev_malloc(&c.work, c.work_bytes));
     c.pinned_doubles = 4096;
     RP_CHECK_HIP(hipHostMalloc((void **)&c.pinned, c.pinned_doubles * sizeof(double),
                                hipHostMallocDefault));
     c.initialized = true;
     return RP_OK;
 }
+
+}  // extern "C"
+
+namespace {
+size_t release_pool();
+}
+
+extern "C" {
 
 int rp_shutdown(void)
 {
@@ -125,6 +136,7 @@ int rp_shutdown(void)
     prof_collect();
     for (auto e : c.event_pool) (void)hipEventDestroy(e);
     c.event_pool.clear();
+    (void)release_pool();
     if (c.work) (void)hipFree(c.work);
     if (c.pinned) (void)hipHostFree(c.pinned);
     if (c.stream) (void)hipStreamDestroy(c.stream);
@@ -149,6 +161,8 @@ int rp_mem_get_info(size_t *free_bytes, size_t *total_bytes)
     if (st) return st;
     RP_REQUIRE(free_bytes && total_bytes, "null output pointer");
     RP_CHECK_HIP(hipMemGetInfo(free_bytes, total_bytes));
+    // blocks kept for reuse count as free: an allocation gets them back
+    *free_bytes += pooled_bytes();
     return RP_OK;
 }
 
@@ -164,6 +178,51 @@ int rp_device_props(int *num_cus, char *name, size_t name_len)
     return RP_OK;
 }
 
+}  // extern "C"
+
+// Blocks of rp_malloc are kept on rp_free, by size, and handed out again to
+// an rp_malloc of the same size: freeing device memory makes the driver
+// clear it before it can be allocated again, which for the tens of 8.6 GB
+// Krylov vectors of one 30-qubit evolution costs seconds per call.  The
+// pool is returned to HIP whenever an allocation runs out of memory, and
+// at rp_shutdown.
+namespace {
+std::mutex g_mem_mu;
+std::unordered_map<void *, size_t> g_live;        // rp_malloc blocks in use
+std::multimap<size_t, void *> g_pool;             // freed blocks by size
+size_t g_pool_bytes = 0;
+
+size_t release_pool()
+{
+    std::lock_guard<std::mutex> lk(g_mem_mu);
+    const size_t n = g_pool_bytes;
+    for (auto &kv : g_pool) (void)hipFree(kv.second);
+    g_pool.clear();
+    g_pool_bytes = 0;
+    return n;
+}
+}  // namespace
+
+namespace rp {
+hipError_t dev_malloc(void **ptr, size_t bytes)
+{
+    hipError_t e = hipMalloc(ptr, bytes);
+    if (e == hipErrorOutOfMemory) {
+        (void)hipGetLastError();
+        if (release_pool() > 0) e = hipMalloc(ptr, bytes);
+    }
+    return e;
+}
+
+size_t pooled_bytes()
+{
+    std::lock_guard<std::mutex> lk(g_mem_mu);
+    return g_pool_bytes;
+}
+}  // namespace rp
+
+extern "C" {
+
 int rp_malloc(void **ptr, size_t bytes)
 {
     int st = check_init();
@@ -171,7 +230,18 @@ int rp_malloc(void **ptr, size_t bytes)
     RP_REQUIRE(ptr, "null output pointer");
     *ptr = nullptr;
     if (bytes == 0) return RP_OK;
-    hipError_t e = hipMalloc(ptr, bytes);
+    {
+        std::lock_guard<std::mutex> lk(g_mem_mu);
+        auto it = g_pool.find(bytes);
+        if (it != g_pool.end()) {
+            *ptr = it->second;
+            g_pool.erase(it);
+            g_pool_bytes -= bytes;
+            g_live[*ptr] = bytes;
+            return RP_OK;
+        }
+    }
+    hipError_t e = dev_malloc(ptr, bytes);
     if (e == hipErrorOutOfMemory || (e == hipSuccess && *ptr == nullptr)) {
         (void)hipGetLastError();
         *ptr = nullptr;
@@ -179,6 +249,8 @@ int rp_malloc(void **ptr, size_t bytes)
                          "Failed to allocate %zu bytes on GPU device.", bytes);
     }
     RP_CHECK_HIP(e);
+    std::lock_guard<std::mutex> lk(g_mem_mu);
+    g_live[*ptr] = bytes;
     return RP_OK;
 }
 
@@ -187,8 +259,19 @@ int rp_free(void *ptr)
     int st = check_init();
     if (st) return st;
     if (!ptr) return RP_OK;
-    // work queued on the library stream may still read the buffer
+    // work queued on the library stream may still read the buffer; after
+    // this wait a reuse of the block is ordered after it
     RP_CHECK_HIP(hipStreamSynchronize(ctx().stream));
+    {
+        std::lock_guard<std::mutex> lk(g_mem_mu);
+        auto it = g_live.find(ptr);
+        if (it != g_live.end()) {
+            g_pool.emplace(it->second, ptr);
+            g_pool_bytes += it->second;
+            g_live.erase(it);
+            return RP_OK;
+        }
+    }
     RP_CHECK_HIP(hipFree(ptr));
     return RP_OK;
 }
