@@ -170,3 +170,35 @@ def test_gs_pass_fused_normalization(gpu, dtype, k):
             assert abs(nrm.value - np.linalg.norm(wgot)) <= 1e-12 * nrm.value
             assert np.abs(wgot - wref).max() <= tol * (1 + np.abs(href).sum())
         dw.free()
+
+
+def test_freed_blocks_are_reused(gpu):
+    """rp_free keeps a block for the next rp_malloc of the same size (no
+    driver clear of freed memory in between), counts it as free memory, and
+    hands out fresh memory for other sizes; data written through the reused
+    block reads back intact."""
+    hip = gpu.hip()
+    nbytes = (48 << 20) + 8 * 1237      # a size no other test allocates
+    p = ctypes.c_void_p()
+    gpu.check(hip.rp_malloc(ctypes.byref(p), nbytes))
+    first = p.value
+    free0, total = ctypes.c_size_t(), ctypes.c_size_t()
+    gpu.check(hip.rp_mem_get_info(ctypes.byref(free0), ctypes.byref(total)))
+    gpu.check(hip.rp_free(p))
+    free1 = ctypes.c_size_t()
+    gpu.check(hip.rp_mem_get_info(ctypes.byref(free1), ctypes.byref(total)))
+    # the kept block counts as free again
+    assert free1.value >= free0.value + nbytes - (2 << 20)
+    q = ctypes.c_void_p()
+    gpu.check(hip.rp_malloc(ctypes.byref(q), nbytes))
+    assert q.value == first
+    r = ctypes.c_void_p()
+    gpu.check(hip.rp_malloc(ctypes.byref(r), nbytes + 4096))
+    assert r.value not in (0, first)
+    a = np.arange(nbytes // 8, dtype=np.float64)
+    gpu.check(hip.rp_memcpy(q, a.ctypes.data_as(ctypes.c_void_p), nbytes, ra.RP_H2D))
+    b = np.empty_like(a)
+    gpu.check(hip.rp_memcpy(b.ctypes.data_as(ctypes.c_void_p), q, nbytes, ra.RP_D2H))
+    assert np.array_equal(a, b)
+    gpu.check(hip.rp_free(q))
+    gpu.check(hip.rp_free(r))
