@@ -149,9 +149,12 @@ k_trace(const typename Cx2<R>::T *__restrict__ mat, uint64_t n, double *__restri
 }
 
 // the rocBLAS handle of the library stream (created on first use; the
-// stream is re-bound on every call since rp_shutdown/rp_init recreate it)
+// stream is re-bound on every call since rp_shutdown/rp_init recreate it).
+// rocBLAS and rocSOLVER allocate their workspaces themselves, so kept device
+// blocks are given back first when fewer than 4 GiB are free.
 static rocblas_handle blas()
 {
+    ensure_device_headroom(size_t(4) << 30);
     static rocblas_handle h = nullptr;
     if (!h && rocblas_create_handle(&h) != rocblas_status_success) h = nullptr;
     if (h) rocblas_set_stream(h, ctx().stream);

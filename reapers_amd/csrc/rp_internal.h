@@ -56,6 +56,10 @@ int set_error(int code, const char *fmt, ...);
 hipError_t dev_malloc(void **ptr, size_t bytes);
 // bytes held in freed blocks kept for reuse
 size_t pooled_bytes();
+// Return the kept blocks to HIP when less than `bytes` of device memory is
+// free: for libraries that allocate on their own (rocBLAS, rocSOLVER) and
+// cannot fall back on the pool.
+void ensure_device_headroom(size_t bytes);
 int check_init();
 
 // launch bracketing for the optional per-class timing

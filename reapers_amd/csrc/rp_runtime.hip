@@ -219,6 +219,17 @@ size_t pooled_bytes()
     std::lock_guard<std::mutex> lk(g_mem_mu);
     return g_pool_bytes;
 }
+
+void ensure_device_headroom(size_t bytes)
+{
+    if (pooled_bytes() == 0) return;
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+        (void)hipGetLastError();
+        return;
+    }
+    if (free_b < bytes) (void)release_pool();
+}
 }  // namespace rp
 
 extern "C" {
