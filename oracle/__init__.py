@@ -270,6 +270,19 @@ def apply(length, bits, coeff, psi, fp=64, out=None):
     return _join(vout)
 
 
+def apply_rows(length, bits, coeff, psi, row_begin, row_end, fp=64):
+    """Rows [row_begin, row_end) of H psi (the reference's CPU apply,
+    inc/cpuimpl.h:737-754, restricted to a window of output rows)."""
+    bits = np.ascontiguousarray(bits, np.uint64)
+    dt = np.float32 if fp == 32 else np.float64
+    c = _split(coeff, dt)
+    vin = _split(psi, dt)
+    vout = np.zeros(2 * row_end, dt)
+    f = lib().or_apply_rows_f32 if fp == 32 else lib().or_apply_rows_f64
+    f(length, _p(bits), _p(c), len(bits), _p(vin), _p(vout), row_begin, row_end)
+    return _join(vout[2 * row_begin:])
+
+
 def apply_rows_timed(length, bits, coeff, psi, row_end, fp=32):
     """Time rows [0, row_end) of out += H psi (the reference's CPU apply) and
     return (seconds, out rows).  psi must already have the fp's dtype layout

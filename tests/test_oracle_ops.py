@@ -178,3 +178,19 @@ def test_product_merge_semantics_cpp_vs_oracle(ra, orc):
                     order.append(bb)
         assert [int(x) for x in bo] == order
         assert np.allclose(co, [merged[x] for x in order], rtol=0, atol=1e-14)
+
+
+def test_oracle_row_windows_match_dense(orc):
+    """apply_rows (the row-window form the full-size GPU tests sample)
+    returns exactly the rows of the Kronecker-product matrix times psi."""
+    rng = np.random.default_rng(9)
+    length = 8
+    bits = rng.integers(0, 1 << (2 * length), size=25).astype(np.uint64)
+    coeff = rng.standard_normal(25) + 1j * rng.standard_normal(25)
+    psi = rng.standard_normal(256) + 1j * rng.standard_normal(256)
+    H = sum(kron_matrix(int(b), c, length) for b, c in zip(bits, coeff))
+    ref = H @ psi
+    for lo, hi in [(0, 256), (0, 1), (37, 101), (255, 256)]:
+        got = orc.apply_rows(length, bits, coeff, psi, lo, hi)
+        assert got.shape == (hi - lo,)
+        assert np.abs(got - ref[lo:hi]).max() < 1e-12 * np.abs(ref).max()
