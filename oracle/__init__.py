@@ -57,7 +57,8 @@ def _bind(L):
                                                _c.c_int64]
         for n in ("or_apply_f64", "or_apply_f32"):
             getattr(L, n).argtypes = [_c.c_int, _vp, _vp, _c.c_int64, _vp, _vp]
-        for n in ("or_apply_rows_f64", "or_apply_rows_f32"):
+        for n in ("or_apply_rows_f64", "or_apply_rows_f32", "or_apply_window_f64",
+                  "or_apply_window_f32"):
             getattr(L, n).argtypes = [_c.c_int, _vp, _vp, _c.c_int64, _vp, _vp, _c.c_int64,
                                       _c.c_int64]
         L.or_dense_matrix_f64.argtypes = [_c.c_int, _vp, _vp, _c.c_int64, _vp]
@@ -272,15 +273,21 @@ def apply(length, bits, coeff, psi, fp=64, out=None):
 
 def apply_rows(length, bits, coeff, psi, row_begin, row_end, fp=64):
     """Rows [row_begin, row_end) of H psi (the reference's CPU apply,
-    inc/cpuimpl.h:737-754, restricted to a window of output rows)."""
+    inc/cpuimpl.h:737-754, restricted to a window of output rows).  A psi
+    already of the fp's complex dtype is read in place (no copy of a large
+    state)."""
     bits = np.ascontiguousarray(bits, np.uint64)
     dt = np.float32 if fp == 32 else np.float64
+    ct = np.complex64 if fp == 32 else np.complex128
     c = _split(coeff, dt)
-    vin = _split(psi, dt)
-    vout = np.zeros(2 * row_end, dt)
-    f = lib().or_apply_rows_f32 if fp == 32 else lib().or_apply_rows_f64
+    if psi.dtype == ct and psi.flags.c_contiguous:
+        vin = psi.view(dt)
+    else:
+        vin = _split(psi, dt)
+    vout = np.zeros(2 * (row_end - row_begin), dt)
+    f = lib().or_apply_window_f32 if fp == 32 else lib().or_apply_window_f64
     f(length, _p(bits), _p(c), len(bits), _p(vin), _p(vout), row_begin, row_end)
-    return _join(vout[2 * row_begin:])
+    return _join(vout)
 
 
 def apply_rows_timed(length, bits, coeff, psi, row_end, fp=32):

@@ -97,6 +97,12 @@ void or_apply_f64(int len, const uint64_t *bits, const double *coeff, int64_t nt
                   const double *in, double *out);
 void or_apply_f32(int len, const uint64_t *bits, const float *coeff, int64_t nterms,
                   const float *in, float *out);
+/* rows [row_begin, row_end) only, out holding just those rows (out[0] is
+   row row_begin): windows of one apply of a state too large to apply whole */
+void or_apply_window_f64(int len, const uint64_t *bits, const double *coeff, int64_t nterms,
+                         const double *in, double *out, int64_t row_begin, int64_t row_end);
+void or_apply_window_f32(int len, const uint64_t *bits, const float *coeff, int64_t nterms,
+                         const float *in, float *out, int64_t row_begin, int64_t row_end);
 /* rows [row_begin, row_end) only: a bounded sample of one apply, for timing */
 void or_apply_rows_f64(int len, const uint64_t *bits, const double *coeff, int64_t nterms,
                        const double *in, double *out, int64_t row_begin, int64_t row_end);

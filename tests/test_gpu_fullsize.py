@@ -6,7 +6,8 @@ last tiles, a window across the middle and random windows -- for FP64, FP32
 with FP64 sums and the benchmark's FP32 upload (autotuned tiling, group
 accumulation); then size-independent properties of the whole state:
 linearity of the apply, Hermiticity <phi|H psi> = conj(<psi|H phi>), the
-norm kept by exp(-iHt) and its time reversal.
+norm kept by exp(-iHt) and its time reversal.  And the OTOC workload's
+size, N=60 (30 qubits), on row windows against the oracle's FP32 apply.
 
 Tolerances: rows FP64 1e-11, FP32 1e-5 (FP64 sums) and 4e-7 (group
 accumulation) of the window's largest amplitude, the bounds of the smaller
@@ -125,3 +126,43 @@ def test_fullsize_evolution_norm_and_time_reversal(gpu, ra, orc):
     H.close()
     assert abs(n1 - 1.0) < 1e-6
     assert dev < 1e-5
+
+
+def test_otoc_size_row_windows_match_oracle(gpu, orc):
+    """30 qubits (sparse SYK N=60, the OTOC workload's 8.6 GB complex<float>
+    state, indices up to 2^30): H psi with the workload's group-accumulating
+    upload, compared on windows of rows (one at the top of the index range)
+    with the oracle's FP32 apply, the reference's FP32 CPU arithmetic (an
+    FP64 copy of the state would take 17 GB of host memory)."""
+    L60 = 30
+    dim = 1 << L60
+    bits, coeff = orc.syk_terms(60, 8.0, seed=4000, fp=32)
+    coeff = coeff.astype(np.complex64).astype(np.complex128)
+    hip = gpu.hip()
+    vin = DeviceBuffer(8 * dim)
+    vout = DeviceBuffer(8 * dim)
+    gpu.check(hip.rp_init_random(gpu.RP_C64, vin.ptr, dim, 4001))
+    psi = vin.to_array(np.complex64, dim)
+    b = np.ascontiguousarray(bits, np.uint64)
+    co = np.empty(2 * len(b))
+    co[0::2], co[1::2] = np.real(coeff), np.imag(coeff)
+    h = ctypes.c_void_p()
+    gpu.check(hip.rp_terms_upload_ex(ctypes.byref(h), gpu.RP_C64, L60,
+                                     b.ctypes.data_as(ctypes.c_void_p),
+                                     co.ctypes.data_as(ctypes.c_void_p), len(b),
+                                     gpu.RP_TERMS_ACC_GROUP))
+    gpu.check(hip.rp_apply_ops(h, vout.ptr, vin.ptr, 0))
+    gpu.check(hip.rp_terms_free(h))
+    vin.free()
+    rng = np.random.default_rng(60)
+    w = 4096
+    starts = [0, dim // 2 - w // 2, dim - w] + [int(s) for s in rng.integers(0, dim - w, 2)]
+    for lo in starts:
+        got = np.empty(w, np.complex64)
+        gpu.check(hip.rp_memcpy(got.ctypes.data_as(ctypes.c_void_p),
+                                ctypes.c_void_p(vout.ptr.value + 8 * lo), 8 * w, gpu.RP_D2H))
+        ref = orc.apply_rows(L60, bits, coeff, psi, lo, lo + w, fp=32)
+        err = np.abs(got.astype(np.complex128) - ref).max() / np.abs(ref).max()
+        print("N=60 rows [%d,%d): %.2e" % (lo, lo + w, err), flush=True)
+        assert err <= 1e-5, (lo, err)
+    vout.free()

@@ -194,3 +194,9 @@ def test_oracle_row_windows_match_dense(orc):
         got = orc.apply_rows(length, bits, coeff, psi, lo, hi)
         assert got.shape == (hi - lo,)
         assert np.abs(got - ref[lo:hi]).max() < 1e-12 * np.abs(ref).max()
+    # FP32 arithmetic on a complex64 state read in place
+    c32 = coeff.astype(np.complex64).astype(np.complex128)
+    p32 = psi.astype(np.complex64)
+    ref32 = sum(kron_matrix(int(b), c, length) for b, c in zip(bits, c32)) @ p32.astype(np.complex128)
+    got = orc.apply_rows(length, bits, c32, p32, 64, 200, fp=32)
+    assert np.abs(got - ref32[64:200]).max() < 1e-5 * np.abs(ref32).max()
