@@ -44,17 +44,21 @@ def prec_of(dtype):
 
 
 class Terms:
-    """rp_terms_t handle from a (bits, coeff) term list."""
+    """rp_terms_t handle from a (bits, coeff) term list; flags as for
+    rp_terms_upload_ex (RP_TERMS_AUTOTUNE, RP_TERMS_ACC_GROUP, ...)."""
 
-    def __init__(self, length, bits, coeff, prec):
+    def __init__(self, length, bits, coeff, prec, flags=0):
         bits = np.ascontiguousarray(bits, np.uint64)
         co = np.empty(2 * len(bits))
         co[0::2] = np.real(coeff)
         co[1::2] = np.imag(coeff)
         self.h = _vp()
-        ra.check(ra.hip().rp_terms_upload(ctypes.byref(self.h), prec, length,
-                                          bits.ctypes.data_as(_vp), co.ctypes.data_as(_vp),
-                                          len(bits)))
+        args = (ctypes.byref(self.h), prec, length, bits.ctypes.data_as(_vp),
+                co.ctypes.data_as(_vp), len(bits))
+        if flags:
+            ra.check(ra.hip().rp_terms_upload_ex(*args, flags))
+        else:
+            ra.check(ra.hip().rp_terms_upload(*args))
         self.length = length
         self.prec = prec
 

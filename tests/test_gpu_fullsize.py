@@ -57,16 +57,8 @@ def test_fullsize_row_windows_match_oracle(gpu, orc, mode):
         coeff = coeff.astype(np.complex64).astype(np.complex128)
     psi = _state(rng, dtype)
     if mode == "fp32-group":
-        b = np.ascontiguousarray(bits, np.uint64)
-        co = np.empty(2 * len(b))
-        co[0::2], co[1::2] = np.real(coeff), np.imag(coeff)
-        h = ctypes.c_void_p()
-        gpu.check(gpu.hip().rp_terms_upload_ex(
-            ctypes.byref(h), gpu.RP_C64, L, b.ctypes.data_as(ctypes.c_void_p),
-            co.ctypes.data_as(ctypes.c_void_p), len(b),
-            gpu.RP_TERMS_AUTOTUNE | gpu.RP_TERMS_ACC_GROUP))
-        terms = Terms.__new__(Terms)
-        terms.h, terms.length, terms.prec = h, L, gpu.RP_C64
+        terms = Terms(L, bits, coeff, gpu.RP_C64,
+                      flags=gpu.RP_TERMS_AUTOTUNE | gpu.RP_TERMS_ACC_GROUP)
         tol = 4e-7
     else:
         terms = Terms(L, bits, coeff, prec_of(dtype))
@@ -143,16 +135,9 @@ def test_otoc_size_row_windows_match_oracle(gpu, orc):
     vout = DeviceBuffer(8 * dim)
     gpu.check(hip.rp_init_random(gpu.RP_C64, vin.ptr, dim, 4001))
     psi = vin.to_array(np.complex64, dim)
-    b = np.ascontiguousarray(bits, np.uint64)
-    co = np.empty(2 * len(b))
-    co[0::2], co[1::2] = np.real(coeff), np.imag(coeff)
-    h = ctypes.c_void_p()
-    gpu.check(hip.rp_terms_upload_ex(ctypes.byref(h), gpu.RP_C64, L60,
-                                     b.ctypes.data_as(ctypes.c_void_p),
-                                     co.ctypes.data_as(ctypes.c_void_p), len(b),
-                                     gpu.RP_TERMS_ACC_GROUP))
-    gpu.check(hip.rp_apply_ops(h, vout.ptr, vin.ptr, 0))
-    gpu.check(hip.rp_terms_free(h))
+    terms = Terms(L60, bits, coeff, gpu.RP_C64, flags=gpu.RP_TERMS_ACC_GROUP)
+    gpu.check(hip.rp_apply_ops(terms.h, vout.ptr, vin.ptr, 0))
+    terms.free()
     vin.free()
     rng = np.random.default_rng(60)
     w = 4096
