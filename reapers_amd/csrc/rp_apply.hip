@@ -173,8 +173,7 @@ __device__ inline void tile_dma(const T *__restrict__ in, uint64_t src, T *dst)
     constexpr int TILE_BYTES = (1 << TB) * (int)sizeof(T);
     constexpr int PER_THREAD = TILE_BYTES / (16 * NT);
     static_assert(PER_THREAD >= 1 && TILE_BYTES % (16 * NT) == 0, "tile too small for DMA");
-    // uniform base plus a 32-bit lane offset: the scalar-base addressing
-    // form, no 64-bit address arithmetic per lane
+    // uniform tile base plus a 32-bit lane offset (below 64 KiB)
     const char *g = (const char *)(in + src);
     char *l = (char *)dst;
     // wave-uniform: kept in a scalar register, so that the LDS address

@@ -120,26 +120,28 @@ def test_fullsize_evolution_norm_and_time_reversal(gpu, ra, orc):
     assert dev < 1e-5
 
 
-def test_otoc_size_row_windows_match_oracle(gpu, orc):
+@pytest.mark.parametrize("NM,seed", [(60, 4000), (62, 6200)])
+def test_otoc_size_row_windows_match_oracle(gpu, orc, NM, seed):
     """30 qubits (sparse SYK N=60, the OTOC workload's 8.6 GB complex<float>
-    state, indices up to 2^30): H psi with the workload's group-accumulating
-    upload, compared on windows of rows (one at the top of the index range)
-    with the oracle's FP32 apply, the reference's FP32 CPU arithmetic (an
-    FP64 copy of the state would take 17 GB of host memory)."""
-    L60 = 30
+    state, indices up to 2^30) and 31 (N=62, 17 GB, indices past the signed
+    32-bit range): H psi with the workload's group-accumulating upload,
+    compared on windows of rows (one at the top of the index range) with the
+    oracle's FP32 apply, the reference's FP32 CPU arithmetic (an FP64 copy
+    of the state would double the host memory)."""
+    L60 = NM // 2
     dim = 1 << L60
-    bits, coeff = orc.syk_terms(60, 8.0, seed=4000, fp=32)
+    bits, coeff = orc.syk_terms(NM, 8.0, seed=seed, fp=32)
     coeff = coeff.astype(np.complex64).astype(np.complex128)
     hip = gpu.hip()
     vin = DeviceBuffer(8 * dim)
     vout = DeviceBuffer(8 * dim)
-    gpu.check(hip.rp_init_random(gpu.RP_C64, vin.ptr, dim, 4001))
+    gpu.check(hip.rp_init_random(gpu.RP_C64, vin.ptr, dim, seed + 1))
     psi = vin.to_array(np.complex64, dim)
     terms = Terms(L60, bits, coeff, gpu.RP_C64, flags=gpu.RP_TERMS_ACC_GROUP)
     gpu.check(hip.rp_apply_ops(terms.h, vout.ptr, vin.ptr, 0))
     terms.free()
     vin.free()
-    rng = np.random.default_rng(60)
+    rng = np.random.default_rng(NM)
     w = 4096
     starts = [0, dim // 2 - w // 2, dim - w] + [int(s) for s in rng.integers(0, dim - w, 2)]
     for lo in starts:
@@ -148,6 +150,6 @@ def test_otoc_size_row_windows_match_oracle(gpu, orc):
                                 ctypes.c_void_p(vout.ptr.value + 8 * lo), 8 * w, gpu.RP_D2H))
         ref = orc.apply_rows(L60, bits, coeff, psi, lo, lo + w, fp=32)
         err = np.abs(got.astype(np.complex128) - ref).max() / np.abs(ref).max()
-        print("N=60 rows [%d,%d): %.2e" % (lo, lo + w, err), flush=True)
+        print("N=%d rows [%d,%d): %.2e" % (NM, lo, lo + w, err), flush=True)
         assert err <= 1e-5, (lo, err)
     vout.free()
