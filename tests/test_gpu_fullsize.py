@@ -120,11 +120,12 @@ def test_fullsize_evolution_norm_and_time_reversal(gpu, ra, orc):
     assert dev < 1e-5
 
 
-@pytest.mark.parametrize("NM,seed", [(60, 4000), (62, 6200)])
+@pytest.mark.parametrize("NM,seed", [(60, 4000), (62, 6200), (64, 6400)])
 def test_otoc_size_row_windows_match_oracle(gpu, orc, NM, seed):
     """30 qubits (sparse SYK N=60, the OTOC workload's 8.6 GB complex<float>
-    state, indices up to 2^30) and 31 (N=62, 17 GB, indices past the signed
-    32-bit range): H psi with the workload's group-accumulating upload,
+    state, indices up to 2^30), 31 (N=62, 17 GB, indices past the signed
+    32-bit range) and 32, the largest chain the API takes (N=64, 34 GB):
+    H psi with the workload's group-accumulating upload,
     compared on windows of rows (one at the top of the index range) with the
     oracle's FP32 apply, the reference's FP32 CPU arithmetic (an FP64 copy
     of the state would double the host memory)."""
