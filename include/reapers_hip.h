@@ -175,6 +175,9 @@ int rp_apply_ops(rp_terms_t h, void *res, const void *vec, int accumulate);
 /* Krylov::orthogonalize (inc/algo.h:76-107) and the answer update of      */
 /* Krylov::evolve (inc/algo.h:336-345)                                     */
 /* ---------------------------------------------------------------------- */
+/* the largest number of basis vectors one fused sweep takes (k below); it
+ * bounds the Krylov dimension of an evolution */
+#define RP_MAX_SWEEP_VECS 64
 /* One classical Gram-Schmidt pass of w against V[0..k-1] (device pointers):
  *   h[i] = <V[i]|w> (FP64), w -= sum_i h[i] V[i], *norm = ||w|| (FP64).
  * h receives 2k doubles (re, im).  k may be 0 (then only the norm). */

@@ -207,6 +207,8 @@ def host():
                                     _c.c_char_p, _i64, _c.POINTER(_i64)], _c.c_int),
             "rph_format_results": ([_dbl, _vp, _i64, _c.c_char_p, _i64, _c.POINTER(_i64)],
                                    _c.c_int),
+            "rph_cross_backend_evolve": ([_c.c_int, _c.c_int, _vp, _vp, _i64, _vp, _dbl, _dbl,
+                                          _c.c_int, _dbl, _vp, _vp, _vp], _c.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(lib, name)
@@ -605,6 +607,21 @@ class BlockState:
             self.close()
         except Exception:
             pass
+
+
+def cross_backend_evolve(length, bits, coeff, psi, t, beta=0.0, krydim=10, tol=0.0, fp=64):
+    """psi as a State on the CPU backend, converted to a GPU State, both
+    evolved by exp(-(beta + i t) H): returns (cpu result, gpu result, the GPU
+    state converted back before the evolution)."""
+    dt = np.complex64 if fp == 32 else np.complex128
+    bits = np.ascontiguousarray(bits, np.uint64)
+    c = np.asarray(coeff, np.complex128)
+    co = np.ascontiguousarray(np.stack([c.real, c.imag], 1).ravel())
+    v = np.ascontiguousarray(psi, dt)
+    outs = [np.empty_like(v) for _ in range(3)]
+    hcheck(host().rph_cross_backend_evolve(fp, length, _ptr(bits), _ptr(co), len(bits), _ptr(v),
+                                           t, beta, krydim, tol, *[_ptr(o) for o in outs]))
+    return tuple(outs)
 
 
 def syk_block_npoint(N, kind="otoc", sp=0.0, J=1.0, seed=0, regularize=False, otoc_idx=-1,

@@ -4,6 +4,8 @@
                        include/reapers_hip.h (hipcc, one object per .hip file)
 - libreapers_host.so : the header-only C++ API instantiated for Python callers
                        (g++, links libreapers_hip.so)
+- libreapers_cpu.so  : the same API built with REAPERS_NOGPU (the CPUImpl
+                       backend) behind include/reapers_cpu.h (g++, no HIP)
 - reapers_service    : the JSON request/response program (g++, links
                        libreapers_hip.so)
 - examples/*         : example programs of the C++ API (g++)
@@ -95,6 +97,22 @@ def build_host(force=False):
     return lib
 
 
+def build_cpu(force=False):
+    """libreapers_cpu.so: the C++ API with REAPERS_NOGPU (CPUImpl backend) behind
+    include/reapers_cpu.h; no HIP library is linked."""
+    deps = (_headers(os.path.join(HERE, "include"))
+            + [os.path.join(ROOT, "include", "reapers_hip.h"),
+               os.path.join(ROOT, "include", "reapers_cpu.h")])
+    src = os.path.join(CSRC, "cpu_api.cc")
+    lib = os.path.join(HERE, "libreapers_cpu.so")
+    if force or _stale(lib, [src] + deps):
+        tmp = lib + ".tmp"
+        flags = [f for f in HOST_FLAGS if f not in ("-O2", "-ffp-contract=off")]
+        _run([CXX, "-O3"] + flags + ["-shared", "-o", tmp, src])
+        os.replace(tmp, lib)
+    return lib
+
+
 def build_service(force=False):
     """reapers_service: the JSON request/response program (csrc/service.cc)."""
     deps = (_headers(os.path.join(HERE, "include"))
@@ -144,6 +162,7 @@ def build_oracle(force=False):
 def build_all(force=False):
     build_hip(force)
     build_host(force)
+    build_cpu(force)
     build_service(force)
     build_examples(force)
     build_oracle(force)

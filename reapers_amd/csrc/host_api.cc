@@ -1223,3 +1223,43 @@ int rph_set_dense_apply(int on)
 }
 
 }  // extern "C"
+
+extern "C" {
+
+// The same state on both backends (inc/state.h:114-118): the host vector psi
+// becomes a State<Fp, CPUImpl>, is converted to a State<Fp, GPUImpl>, and
+// both are evolved by exp(-(beta + i t) H) with their own backend (Krylov
+// dimension krydim, tolerance tol).  out_cpu receives the CPU result,
+// out_gpu the GPU result converted back into a CPU state, and out_copy the
+// GPU state converted back before the evolution (a round trip of the
+// conversions).  Vectors are complex<Fp> of 2^len elements.
+int rph_cross_backend_evolve(int fp, int len, const uint64_t *bits, const double *coeff,
+                             int64_t nterms, const void *psi, double t, double beta, int krydim,
+                             double tol, void *out_cpu, void *out_gpu, void *out_copy)
+{
+    return guarded([&] {
+        auto run = [&](auto tag) {
+            using Fp = decltype(tag);
+            HostSumOps<Fp> hops = HostSumOps<Fp>::from_terms(bits, coeff, (size_t)nterms);
+            DevSumOps<Fp> dops(hops);
+            State<Fp, CPUImpl> c(len);
+            c.load((const complex<Fp> *)psi);
+            State<Fp, GPUImpl> g(c);
+            State<Fp, CPUImpl> back(g);
+            back.store((complex<Fp> *)out_copy);
+            KrylovOptions opt;
+            opt.krydim = krydim;
+            opt.tol = tol;
+            c.evolve_ex(hops, t, beta, opt);
+            g.evolve_ex(dops, t, beta, opt);
+            c.store((complex<Fp> *)out_cpu);
+            State<Fp, CPUImpl> gc(g);
+            gc.store((complex<Fp> *)out_gpu);
+        };
+        if (fp == 32) run(float{});
+        else if (fp == 64) run(double{});
+        else ThrowException(InvalidArgument, "fp", "must be 32 or 64");
+    });
+}
+
+}  // extern "C"

@@ -839,6 +839,10 @@ int rp_terms_set_tiling(rp_terms_t h, int tile_bits, int threads, int depth)
     if (st) return st;
     RP_REQUIRE(h, "null term handle");
     RP_REQUIRE(tile_bits <= h->len, "tile bits exceed the chain length");
+    // the one-thread-per-amplitude kernels index 32-bit amplitudes and the
+    // sliced one keeps depth * 2^len partial sums
+    RP_REQUIRE(tile_bits != 0 || h->len <= kSplitMaxLen,
+               "tile bits 0 (one thread per amplitude) only up to len 16");
     RP_REQUIRE(tiling_instantiated(h->prec, tile_bits, threads, depth),
                "tiling not instantiated for this precision");
     RP_CHECK_HIP(hipStreamSynchronize(ctx().stream));

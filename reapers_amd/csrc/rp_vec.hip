@@ -29,7 +29,7 @@ static inline int grid_for(uint64_t units, int threads, int cap)
 
 constexpr int kEwThreads = 256;
 constexpr int kEwMaxBlocks = 65536;
-constexpr int kMaxVecs = 64;      // vectors per fused sweep (API limit)
+constexpr int kMaxVecs = RP_MAX_SWEEP_VECS;   // vectors per fused sweep (API limit)
 constexpr int kDotChunk = 8;      // vectors whose dots one launch accumulates
 constexpr int kUpdChunk = 16;     // vectors one update / lincomb launch reads
 constexpr int kWaves = kRedThreads / 64;

@@ -42,7 +42,7 @@ class Krylov {
         const double eta = 1 / std::sqrt(2.0);
         const size_t dim = s.dim();
         std::vector<const typename Impl::template VecType<FpType> *> V(j + 1);
-        for (int i = 0; i <= j; i++) V[i] = &s.buf(i);
+        for (int i = 0; i <= j; i++) V[i] = &s.vec(i);
         std::vector<cd> h(j + 1, cd(0)), r;
         double old_norm = 1.0, norm = eta / 2;
         int niter = 0;
@@ -157,7 +157,7 @@ class Krylov {
             st.host_ms += std::chrono::duration<double, std::milli>(
                               std::chrono::steady_clock::now() - t0).count();
             std::vector<const typename Impl::template VecType<FpType> *> V(m);
-            for (int i = 0; i < m; i++) V[i] = &s.buf(i);
+            for (int i = 0; i < m; i++) V[i] = &s.vec(i);
             Impl::lincomb(dim, s.buf(ans), V, c);
             const double nrm = std::sqrt(nrm2);
             st.last_nrm = nrm;
@@ -203,6 +203,12 @@ public:
     static void evolve_ex(State<FpType, Impl> &s, const SumOps<FpType> &ops, double t_real,
                           double t_imag, const KrylovOptions &opt, KrylovStats &st) {
         if (opt.krydim < 2) ThrowException(InvalidArgument, "krydim", "must be at least two");
+        // the last Arnoldi step's Gram-Schmidt sweep and the answer update
+        // each take krydim basis vectors in one fused sweep
+        if (opt.krydim > RP_MAX_SWEEP_VECS)
+            ThrowException(InvalidArgument, "krydim", "must be at most 64");
+        if (opt.schedule && opt.max_krydim > RP_MAX_SWEEP_VECS)
+            ThrowException(InvalidArgument, "max_krydim", "must be at most 64");
         RestartInfo ri;
         ri.len = s.spin_chain_length();
         ri.nterms = (long)ops.size();
@@ -269,7 +275,7 @@ public:
             std::vector<cd> c(krydim);
             for (int i = 0; i < krydim; i++) c[i] = Q[(size_t)idx * krydim + i];
             std::vector<const typename Impl::template VecType<FpType> *> V(krydim);
-            for (int i = 0; i < krydim; i++) V[i] = &s.buf(i);
+            for (int i = 0; i < krydim; i++) V[i] = &s.vec(i);
             Impl::zero_vec(dim, s.buf(krydim));
             Impl::lincomb(dim, s.buf(krydim), V, c);
             s.inc_curbuf(krydim);

@@ -113,6 +113,20 @@ struct BlockDiag {
                             RR(std::move(RR)) {}
     BlockDiag(const BlockDiag &) = default;
     BlockDiag(BlockDiag &&) = default;
+    // blocks of another type, e.g. another precision (ham64 = ham32 in the
+    // production driver, examples/syk.cc:457-462)
+    template<typename T1>
+        requires(!std::is_same_v<T1, T>)
+    BlockDiag(const BlockDiag<T1> &b)
+        : nullLL(b.nullLL), nullRR(b.nullRR), LL(b.nullLL ? T{} : T(b.LL)),
+          RR(b.nullRR ? T{} : T(b.RR)) {}
+    template<typename T1>
+        requires(!std::is_same_v<T1, T>)
+    BlockDiag &operator=(const BlockDiag<T1> &b) {
+        blk::assign_block(nullLL, LL, b.nullLL, b.LL);
+        blk::assign_block(nullRR, RR, b.nullRR, b.RR);
+        return *this;
+    }
     BlockDiag &operator=(const BlockDiag &b) {
         blk::assign_block(nullLL, LL, b.nullLL, b.LL);
         blk::assign_block(nullRR, RR, b.nullRR, b.RR);

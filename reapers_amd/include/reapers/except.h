@@ -32,6 +32,16 @@ public:
     }
 };
 
+class IndexTooSmall : public Exception {
+public:
+    IndexTooSmall(const char *f, int l, const char *fn, size_t n, size_t nmin, const char *ty)
+        : Exception(f, l, fn) {
+        std::stringstream ss;
+        ss << "Specified " << ty << " index " << n << " cannot be less than " << nmin << ".";
+        msg += ss.str();
+    }
+};
+
 class SpinIndexTooLarge : public IndexTooLarge {
 public:
     SpinIndexTooLarge(const char *f, int l, const char *fn, size_t n, size_t nmax)
@@ -48,6 +58,18 @@ class StateIndexTooLarge : public IndexTooLarge {
 public:
     StateIndexTooLarge(const char *f, int l, const char *fn, size_t n, size_t nmax)
         : IndexTooLarge(f, l, fn, n, nmax, "state") {}
+};
+
+class SpinIndexTooSmall : public IndexTooSmall {
+public:
+    SpinIndexTooSmall(const char *f, int l, const char *fn, size_t n, size_t nmin)
+        : IndexTooSmall(f, l, fn, n, nmin, "spin") {}
+};
+
+class FieldIndexTooSmall : public IndexTooSmall {
+public:
+    FieldIndexTooSmall(const char *f, int l, const char *fn, size_t n, size_t nmin)
+        : IndexTooSmall(f, l, fn, n, nmin, "field") {}
 };
 
 class InvalidArgument : public Exception {
@@ -95,6 +117,18 @@ public:
         : Exception(f, l, fn) {
         msg += "No GPU device available";
         if (detail && *detail) { msg += ": "; msg += detail; }
+    }
+};
+
+// peer access between two devices is unavailable (the reference's
+// multi-GPU split; this backend runs on one device and never throws it)
+class NoP2PAccess : public Exception {
+public:
+    NoP2PAccess(const char *f, int l, const char *fn, int dev0, int dev1) : Exception(f, l, fn) {
+        std::stringstream ss;
+        ss << "P2P access between device " << dev0 << " and device " << dev1
+           << " is not supported.";
+        msg += ss.str();
     }
 };
 

@@ -35,6 +35,21 @@ public:
     SubspaceView &operator=(const SubspaceView &) = default;
     SubspaceView &operator=(SubspaceView &&) = default;
 
+    // a view over term lists of another precision (inc/fermions.h:62-72):
+    // the production driver promotes its FP32 Hamiltonian with ham64 = ham32
+    template<typename S1>
+        requires(!std::is_same_v<S1, SumOpsTy>)
+    SubspaceView(const SubspaceView<S1> &view)
+        : SumOpsTy(static_cast<const S1 &>(view)), len(view.len) {}
+
+    template<typename S1>
+        requires(!std::is_same_v<S1, SumOpsTy>)
+    SubspaceView &operator=(const SubspaceView<S1> &view) {
+        SumOpsTy::operator=(static_cast<const S1 &>(view));
+        len = view.len;
+        return *this;
+    }
+
     static auto identity(int len) {
         return SubspaceView<SumOpsTy>(len, SpinOp<typename SumOpsTy::RealScalarType>::identity());
     }
@@ -113,7 +128,11 @@ public:
     int spin_chain_length() const { return len; }
 
     // dense forms on this view's chain length (inc/fermions.h:183-203)
+    using MatrixType = typename SumOpsTy::MatrixType;
     decltype(auto) get_matrix() const { return SumOpsTy::get_matrix(len); }
+    // the view reads as its dense matrix (Matrix A = syk.fermion_ops(n))
+    operator const MatrixType &() const { return SumOpsTy::get_matrix(len); }
+    MatrixType operator*(const MatrixType &m) const { return SumOpsTy::get_matrix(len) * m; }
     decltype(auto) get_eigensystem() const { return SumOpsTy::get_eigensystem(len); }
     decltype(auto) matexp(typename SumOpsTy::RealScalarType c) const {
         return SumOpsTy::matexp(typename SumOpsTy::ComplexScalarType(c), len);

@@ -12,7 +12,9 @@
 //     truncates, src/krnl.cu:33-39).
 // Every call is checked and throws the reference's exception types.  There is
 // no CPU fallback: if the HIP library cannot initialize a device, the first
-// use throws NoGpuDevice.
+// use throws NoGpuDevice.  CPUImpl (cpuimpl.h) is a separate backend that a
+// program selects (REAPERS_NOGPU, or State<F, CPUImpl>); states convert
+// between the two explicitly.
 #pragma once
 
 #ifndef _REAPERS_H_
@@ -52,6 +54,7 @@ class DevSumOps : public HostSumOps<FpType> {
     }
 
     void release() const override {
+        HostSumOps<FpType>::release();
         free_handle();
         dense.reset();
     }
@@ -176,6 +179,17 @@ public:
             }
         };
         ElemRefType operator[](size_t idx) { return ElemRefType(*this, idx); }
+        // conversions to and from the CPU backend's vectors (DeviceVec's
+        // CPUImpl::VecType constructor and conversion operator,
+        // inc/gpuimpl.h:438-462): one host<->device copy each
+        explicit DeviceVec(const CPUImpl::HostVec<FpType> &v) : DeviceVec(v.dimension()) {
+            upload(v.get());
+        }
+        explicit operator CPUImpl::HostVec<FpType>() const {
+            CPUImpl::HostVec<FpType> v(dim);
+            download(v.get());
+            return v;
+        }
         // host <-> device transfers of the whole vector
         void upload(const complex<FpType> *host) {
             RP_CALL(rp_memcpy(ptr, host, dim * sizeof(complex<FpType>), RP_H2D));

@@ -24,6 +24,33 @@ using u8 = std::uint8_t;
 using u32 = std::uint32_t;
 using u64 = std::uint64_t;
 
+// imaginary-unit literals (inc/ops.h:47-72): 2_i is i*2 in DefFpType,
+// _if in float, _id in double
+constexpr complex<DefFpType> operator""_i(unsigned long long d)
+{
+    return complex<DefFpType>{0, static_cast<DefFpType>(d)};
+}
+constexpr complex<DefFpType> operator""_i(long double d)
+{
+    return complex<DefFpType>{0, static_cast<DefFpType>(d)};
+}
+constexpr complex<float> operator""_if(unsigned long long d)
+{
+    return complex<float>{0, static_cast<float>(d)};
+}
+constexpr complex<float> operator""_if(long double d)
+{
+    return complex<float>{0, static_cast<float>(d)};
+}
+constexpr complex<double> operator""_id(unsigned long long d)
+{
+    return complex<double>{0, static_cast<double>(d)};
+}
+constexpr complex<double> operator""_id(long double d)
+{
+    return complex<double>{0, static_cast<double>(d)};
+}
+
 namespace internal {
 constexpr u64 kEvenBits = 0x5555555555555555ull;
 // gather bits 2s (X) or 2s+1 (Z) of a SpinOp bit string into bit s

@@ -2,9 +2,13 @@
 //
 // Mirrors the reference's inc/reapers.h:22-123: namespace REAPERS, DefFpType
 // (float unless REAPERS_FP64), complex = std::complex, SumOps = the backend's
-// term list, State, EvolutionAlgorithm::Krylov, Model::{FermionOp, SYK}.  The
-// backend is GPUImpl over libreapers_hip.so's C ABI; link with
-// -lreapers_hip.  Host code only: no HIP headers are needed to use it.
+// term list, State, EvolutionAlgorithm::Krylov, Model::{FermionOp, SYK}.
+// The default backend (DefImpl) is GPUImpl over libreapers_hip.so's C ABI;
+// link with -lreapers_hip.  Defining REAPERS_NOGPU selects CPUImpl instead
+// (inc/reapers.h:96-100), and such a program needs neither the HIP library
+// nor a GPU.  CPUImpl is also available next to GPUImpl in a GPU build, with
+// explicit State conversions between the two.  Host code only: no HIP
+// headers are needed to use it.
 #pragma once
 
 #include <algorithm>
@@ -59,13 +63,18 @@ concept ScalarType = RealScalar<T> || ComplexScalar<T>;
 
 #include "except.h"
 #include "ops.h"
-#include "sumops.h"
 #include "expm.h"
+#include "hostmat.h"
+#include "sumops.h"
+#include "cpuimpl.h"
+#ifdef REAPERS_NOGPU
+using DefImpl = CPUImpl;
+#else
 #include "gpuimpl.h"
 #include "dense.h"
-#include "blkops.h"
-
 using DefImpl = GPUImpl;
+#endif
+#include "blkops.h"
 
 template<RealScalar FpType = DefFpType>
 using SumOps = DefImpl::SumOps<FpType>;
@@ -73,7 +82,9 @@ using SumOps = DefImpl::SumOps<FpType>;
 #include "state.h"
 #include "schedule.h"
 #include "otoc.h"
+#ifndef REAPERS_NOGPU
 #include "batch.h"
+#endif
 
 namespace Model {
 #include "fermions.h"

@@ -60,6 +60,7 @@ inline void proposer_features(const RestartInfo &ri, bool for_substeps,
     f[11] = (float)(std::log1p(std::abs(ri.beta0) * tau) / 5);
 }
 
+#ifndef REAPERS_NOGPU
 class MLPSchedule : public KrylovSchedule {
     float *params = nullptr;   // device copy of the RP_MLP_NPARAMS parameters
     int min_dim, max_dim, max_sub;
@@ -101,3 +102,4 @@ public:
         return (int)std::max<long>(min_dim, std::min<long>(m, max_dim));
     }
 };
+#endif  // REAPERS_NOGPU

@@ -116,6 +116,23 @@ public:
         Impl::copy_vec(dim(), buf(), st.buf(), rounding);
     }
 
+    // conversion between backends (inc/state.h:114-118): the current vector
+    // is copied into the other backend's memory; a precision change, if
+    // any, is made on the source backend first
+    template<RealScalar F1, typename I1>
+        requires(!std::is_same_v<I1, Impl>)
+    explicit State(const State<F1, I1> &st) : len(st.len), curbuf(0) {
+        if (!len) return;
+        if constexpr (std::is_same_v<F1, FpType>) {
+            bufs.emplace_back(static_cast<typename Impl::template VecType<FpType>>(
+                st.bufs[st.curbuf % st.num_bufs()]));
+        } else {
+            State<FpType, I1> conv(st);
+            bufs.emplace_back(static_cast<typename Impl::template VecType<FpType>>(
+                conv.bufs[conv.curbuf % conv.num_bufs()]));
+        }
+    }
+
     void random_state(u64 seed = 0) {
         if (!len) return;
         Impl::init_random(dim(), buf(), seed);
@@ -138,6 +155,13 @@ public:
     typename Impl::template ConstBufType<FpType> buf(int i = 0) const {
         assert(len != 0 && num_bufs() >= i + 1);
         return bufs[(curbuf + i) % num_bufs()].get();
+    }
+
+    // the backend vector object of buf(i) (the fused Krylov sweeps take a
+    // list of them)
+    const typename Impl::template VecType<FpType> &vec(int i = 0) const {
+        assert(len != 0 && num_bufs() >= i + 1);
+        return bufs[(curbuf + i) % num_bufs()];
     }
 
     void inc_curbuf(int i = 1) {
@@ -172,8 +196,18 @@ public:
     }
 
     // whole-vector host transfers (interleaved complex<FpType>)
-    void load(const complex<FpType> *host) { buf().upload(host); }
-    void store(complex<FpType> *host) const { buf().download(host); }
+    void load(const complex<FpType> *host) {
+        if constexpr (std::is_pointer_v<typename Impl::template BufType<FpType>>)
+            std::memcpy(buf(), host, dim() * sizeof(complex<FpType>));
+        else
+            buf().upload(host);
+    }
+    void store(complex<FpType> *host) const {
+        if constexpr (std::is_pointer_v<typename Impl::template BufType<FpType>>)
+            std::memcpy(host, buf(), dim() * sizeof(complex<FpType>));
+        else
+            buf().download(host);
+    }
 
     State &operator=(const State &st) {
         if (this == &st) return *this;
@@ -238,7 +272,7 @@ public:
     State &operator*=(const typename Impl::template SumOps<FpType> &ops) {
         if (!len) return *this;
         enlarge(2);
-        auto &v = buf();
+        auto &&v = buf();
         inc_curbuf();
         Impl::apply_ops_overwrite(len, buf(), ops, v);
         return *this;
@@ -360,6 +394,22 @@ public:
     BlockState(Base &&b) : Base(std::move(b)), slen(len_of(*this)) {}
     BlockState(const BlockState &) = default;
     BlockState(BlockState &&) = default;
+    // another precision or backend, sector by sector (null sectors stay null)
+    template<RealScalar F1, typename I1>
+        requires(!std::is_same_v<F1, FpType> || !std::is_same_v<I1, Impl>)
+    explicit BlockState(const BlockState<F1, I1> &b)
+        : Base(b.nullL ? StateType{} : StateType(b.L), b.nullR ? StateType{} : StateType(b.R)),
+          slen(b.spin_chain_length()) {}
+    template<RealScalar F1>
+        requires(!std::is_same_v<F1, FpType>)
+    BlockState &operator=(const BlockState<F1, Impl> &b) {
+        if (b.nullL) { this->L = StateType{}; this->nullL = true; }
+        else { this->L = b.L; this->nullL = false; }
+        if (b.nullR) { this->R = StateType{}; this->nullR = true; }
+        else { this->R = b.R; this->nullR = false; }
+        slen = b.spin_chain_length();
+        return *this;
+    }
     BlockState &operator=(const Base &b) {
         Base::operator=(b);
         if (len_of(b)) slen = len_of(b);

@@ -26,13 +26,18 @@ public:
 protected:
     std::vector<SpinOp<FpType>> ops;
 
-    // called on every mutation; derived classes drop cached device state
-    virtual void release() const {}
+    // the dense forms of the CPU backend, built on request
+    struct HostDense;
+    mutable std::unique_ptr<HostDense> hdense;
+
+    // called on every mutation: drops the cached dense forms; derived
+    // classes drop their device state too
+    virtual void release() const { hdense.reset(); }
 
 public:
     HostSumOps(const SpinOp<FpType> &op) : ops{op} {}
     HostSumOps(const HostSumOps &op) : ops{op.ops} {}
-    HostSumOps(HostSumOps &&op) = default;
+    HostSumOps(HostSumOps &&op) : ops(std::move(op.ops)) { op.release(); }
 
     template<RealScalar F1>
     explicit HostSumOps(const HostSumOps<F1> &op) {
@@ -204,6 +209,21 @@ public:
         return res;
     }
 
+    // Dense forms on len sites for the CPU backend (get_matrix,
+    // get_eigensystem, matexp: inc/cpuimpl.h:336-420), cached until the
+    // term list changes.  Matrix element <i|H|i ^ x> = sum over the terms
+    // with flip mask x of coeff (-1)^popcount((i ^ x) & z) (SpinOp::apply).
+    using MatrixType = HostMatrix<FpType>;
+    using EigenVals = std::vector<FpType>;
+    using EigenVecs = HostMatrix<FpType>;
+    using EigenSystem = std::tuple<const EigenVals &, const EigenVecs &>;
+    const MatrixType &get_matrix(int len) const;
+    EigenSystem get_eigensystem(int len) const;
+    // exp(c H) = V diag(exp(c w)) V^H, kept per c under the CPU backend's
+    // cache budget, least-used entries dropped first (inc/cpuimpl.h:29-88)
+    const MatrixType &matexp(complex<FpType> c, int len) const;
+    size_t matexp_cache_entries() const;
+
     friend std::ostream &operator<<(std::ostream &os, const HostSumOps &s) {
         bool first = true;
         if (s.ops.empty()) os << "0";
@@ -226,3 +246,110 @@ public:
         return os;
     }
 };
+
+template<RealScalar FpType>
+struct HostSumOps<FpType>::HostDense {
+    int len = -1;
+    std::unique_ptr<HostMatrix<FpType>> mat, vecs;
+    std::vector<FpType> vals;
+    std::vector<double> vals64;
+    std::vector<std::complex<double>> vecs64;
+    struct Entry {
+        HostMatrix<FpType> m;
+        u64 hits;
+    };
+    std::map<std::pair<FpType, FpType>, Entry> expm;
+    ~HostDense() {
+        auto &b = internal::HostCacheBudget::get();
+        for (auto &e : expm) b.used -= std::min(b.used, e.second.m.bytes());
+    }
+};
+
+template<RealScalar FpType>
+const HostMatrix<FpType> &HostSumOps<FpType>::get_matrix(int len) const {
+    if (len < 1 || len > 15) ThrowException(InvalidArgument, "len", "must be 1..15 for dense forms");
+    if (!hdense || hdense->len != len) {
+        hdense = std::make_unique<HostDense>();
+        hdense->len = len;
+    }
+    if (!hdense->mat) {
+        const size_t n = (size_t)1 << len;
+        auto m = std::make_unique<HostMatrix<FpType>>(n, n);
+        for (auto &op : ops) {
+            const u64 b = op.get_bit_string();
+            const u32 x = internal::flip_mask(b), z = internal::phase_mask(b);
+            const complex<FpType> c = op.get_coeff();
+            for (size_t i = 0; i < n; i++) {
+                const size_t col = i ^ x;
+                (*m)(i, col) += (__builtin_popcountll(col & z) & 1) ? -c : c;
+            }
+        }
+        hdense->mat = std::move(m);
+    }
+    return *hdense->mat;
+}
+
+template<RealScalar FpType>
+typename HostSumOps<FpType>::EigenSystem HostSumOps<FpType>::get_eigensystem(int len) const {
+    const auto &m = get_matrix(len);
+    auto &d = *hdense;
+    if (!d.vecs) {
+        const size_t n = m.rows();
+        std::vector<std::complex<double>> a(n * n);
+        for (size_t i = 0; i < n * n; i++) a[i] = std::complex<double>(m.data()[i]);
+        internal::heigh(n, std::move(a), d.vals64, d.vecs64);
+        d.vals.resize(n);
+        for (size_t i = 0; i < n; i++) d.vals[i] = (FpType)d.vals64[i];
+        auto v = std::make_unique<HostMatrix<FpType>>(n, n);
+        for (size_t i = 0; i < n * n; i++)
+            v->data()[i] = complex<FpType>((FpType)d.vecs64[i].real(), (FpType)d.vecs64[i].imag());
+        d.vecs = std::move(v);
+    }
+    return {d.vals, *d.vecs};
+}
+
+template<RealScalar FpType>
+const HostMatrix<FpType> &HostSumOps<FpType>::matexp(complex<FpType> c, int len) const {
+    get_eigensystem(len);
+    auto &d = *hdense;
+    auto key = std::make_pair(c.real(), c.imag());
+    if (auto it = d.expm.find(key); it != d.expm.end()) {
+        it->second.hits++;
+        return it->second.m;
+    }
+    const size_t n = (size_t)1 << len;
+    // V diag(e) V^H in FP64 from the FP64 eigensystem
+    std::vector<std::complex<double>> e(n);
+    const std::complex<double> cc(c.real(), c.imag());
+    for (size_t j = 0; j < n; j++) e[j] = std::exp(cc * d.vals64[j]);
+    HostMatrix<FpType> r(n, n);
+    const long nl = (long)n;
+#pragma omp parallel for schedule(static)
+    for (long col = 0; col < nl; col++) {
+        std::vector<std::complex<double>> acc(n);
+        for (size_t j = 0; j < n; j++) {
+            const std::complex<double> f = e[j] * std::conj(d.vecs64[j * n + (size_t)col]);
+            const std::complex<double> *vj = &d.vecs64[j * n];
+            for (size_t i = 0; i < n; i++) acc[i] += vj[i] * f;
+        }
+        for (size_t i = 0; i < n; i++)
+            r((size_t)i, (size_t)col) = complex<FpType>((FpType)acc[i].real(), (FpType)acc[i].imag());
+    }
+    auto &b = internal::HostCacheBudget::get();
+    const size_t sz = r.bytes();
+    while (b.used + sz > b.max && !d.expm.empty()) {
+        auto victim = d.expm.begin();
+        for (auto it = d.expm.begin(); it != d.expm.end(); ++it)
+            if (it->second.hits < victim->second.hits) victim = it;
+        b.used -= std::min(b.used, victim->second.m.bytes());
+        d.expm.erase(victim);
+    }
+    b.used += sz;
+    auto it = d.expm.insert_or_assign(key, typename HostDense::Entry{std::move(r), 0}).first;
+    return it->second.m;
+}
+
+template<RealScalar FpType>
+size_t HostSumOps<FpType>::matexp_cache_entries() const {
+    return hdense ? hdense->expm.size() : 0;
+}

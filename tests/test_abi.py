@@ -29,6 +29,22 @@ def test_library_exports_every_declared_symbol(ra):
     assert not missing, missing
 
 
+def test_cpu_library_exports_every_declared_symbol(ra):
+    """include/reapers_cpu.h (the REAPERS_NOGPU build of the C++ API) is
+    exported by libreapers_cpu.so, which links no HIP library."""
+    from reapers_amd import cpu
+    text = open(os.path.join(ROOT, "include", "reapers_cpu.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = sorted(set(re.findall(r"\b(rpc_[a-z0-9_]+)\s*\(", text)))
+    assert "rpc_evolve" in names and "rpc_apply" in names
+    lib = ctypes.CDLL(cpu.CPU_LIB)
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    import subprocess
+    deps = subprocess.run(["ldd", cpu.CPU_LIB], capture_output=True, text=True).stdout
+    assert "amdhip" not in deps and "reapers_hip" not in deps
+
+
 def test_host_library_loads(ra):
     ra.host()
 
