@@ -95,6 +95,17 @@ def lib():
     return _lib
 
 
+def fast():
+    """The timing-only build (liboracle_fast.so: -O3 with AVX2/FMA, FMA
+    contraction allowed), for bench.py's CPU baseline; never the checker."""
+    if "fast" not in _variants:
+        so = os.path.join(HERE, "liboracle_fast.so")
+        if not os.path.exists(so):
+            raise RuntimeError("oracle/liboracle_fast.so is not built (make -C oracle)")
+        _variants["fast"] = _bind(_c.CDLL(so))
+    return _variants["fast"]
+
+
 def planted(n):
     """A deliberately wrong build of the oracle (OR_PLANT=n, see oracle.c),
     built on demand; used only to show that the pinning tests catch it."""
