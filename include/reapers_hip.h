@@ -170,6 +170,19 @@ int rp_terms_set_tiling(rp_terms_t h, int tile_bits, int threads, int depth);
  * precision. */
 int rp_apply_ops(rp_terms_t h, void *res, const void *vec, int accumulate);
 
+/* The Arnoldi step's apply fused with its first Gram-Schmidt pass (no
+ * counterpart in the reference, which calls apply_ops and then one vec_prod
+ * per basis vector, inc/algo.h:86-101 and 160): res = H vec, then the pass of
+ * rp_gs_pass_fused over V[0..k) (k <= 64) on res.  Where the handle's layout
+ * allows (complex<float> states with FP64 or group accumulation, 2^12-
+ * amplitude tiles of 512 threads, up to 2^28 amplitudes) the coefficients
+ * <V_q|res> and ||res||^2 are formed in the apply kernel from the values it
+ * writes, saving the pass's separate read of res; otherwise this is
+ * rp_apply_ops followed by rp_gs_pass_fused.  res must differ from vec and
+ * from every V[q]. */
+int rp_apply_gs(rp_terms_t h, void *res, const void *vec, const void *const *V, int k,
+                double *hcoef, double *norm, double fuse_min, int *normalized);
+
 /* ---------------------------------------------------------------------- */
 /* fused Krylov sweeps -- replace the per-vector dot/axpy/norm calls of    */
 /* Krylov::orthogonalize (inc/algo.h:76-107) and the answer update of      */

@@ -311,6 +311,7 @@ def apply_rows_timed(length, bits, coeff, psi, row_end, fp=32):
     c = _split(coeff, dt)
     vin = np.ascontiguousarray(psi).view(dt)
     vout = np.zeros(2 * row_end, dt)
+    vout.fill(0)                # map the output pages before the timed call
     f = lib().or_apply_rows_f32 if fp == 32 else lib().or_apply_rows_f64
     t0 = time.perf_counter()
     f(length, _p(bits), _p(c), len(bits), _p(vin), _p(vout), 0, row_end)

@@ -19,6 +19,7 @@ HIP_LIB = os.path.join(HERE, "libreapers_hip.so")
 HOST_LIB = os.path.join(HERE, "libreapers_host.so")
 
 RP_C64, RP_C128 = 0, 1
+RP_OK, RP_ERR_INVALID_ARGUMENT, RP_ERR_OUT_OF_MEMORY, RP_ERR_RUNTIME, RP_ERR_NO_DEVICE = 0, 1, 2, 3, 4
 RP_ROUND_NEAREST, RP_ROUND_TOWARD_ZERO = 0, 1
 RP_H2D, RP_D2H, RP_D2D = 0, 1, 2
 RP_KERNEL_APPLY, RP_KERNEL_GS, RP_KERNEL_OTHER = 0, 1, 2
@@ -92,6 +93,8 @@ def hip():
             "rp_terms_set_tiling": [_vp, _c.c_int, _c.c_int, _c.c_int],
             "rp_batch_groups": [_vp, _vp],
             "rp_apply_ops": [_vp, _vp, _vp, _c.c_int],
+            "rp_apply_gs": [_vp, _vp, _vp, _vp, _c.c_int, _vp, _c.POINTER(_dbl), _dbl,
+                            _c.POINTER(_c.c_int)],
             "rp_gs_pass": [_c.c_int, _vp, _c.c_int, _vp, _u64, _vp, _c.POINTER(_dbl)],
             "rp_gs_pass_fused": [_c.c_int, _vp, _c.c_int, _vp, _u64, _vp, _c.POINTER(_dbl), _dbl,
                                  _c.POINTER(_c.c_int)],
@@ -200,6 +203,7 @@ def host():
             "rph_ham_matexp_entries": ([_vp, _c.POINTER(_i64)], _c.c_int),
             "rph_state_set": ([_vp, _u64, _dbl, _dbl], _c.c_int),
             "rph_set_dense_apply": ([_c.c_int], _c.c_int),
+            "rph_set_fused_first_pass": ([_c.c_int], _c.c_int),
             "rph_cache_budget": ([_i64, _c.POINTER(_i64), _c.POINTER(_i64)], _c.c_int),
             "rph_ham_to_string": ([_c.c_int, _vp, _vp, _i64, _c.c_char_p, _i64,
                                    _c.POINTER(_i64)], _c.c_int),
@@ -347,6 +351,13 @@ def set_dense_apply(on):
     """Apply through the dense matrix when the term count reaches 2^len (the
     reference's small-problem switch); returns the previous setting."""
     return bool(host().rph_set_dense_apply(int(bool(on))))
+
+
+def set_fused_first_pass(on):
+    """Have the Krylov driver's Arnoldi steps form their first Gram-Schmidt
+    pass's coefficients in the apply kernel (rp_apply_gs); returns the
+    previous setting."""
+    return bool(host().rph_set_fused_first_pass(int(bool(on))))
 
 
 def matexp_cache_budget(max_bytes=None):

@@ -1222,6 +1222,15 @@ int rph_set_dense_apply(int on)
     return prev;
 }
 
+// Krylov's Arnoldi steps through rp_apply_gs (apply + first Gram-Schmidt
+// pass in one kernel); returns the previous setting
+int rph_set_fused_first_pass(int on)
+{
+    int prev = GPUImpl::fused_first_pass() ? 1 : 0;
+    GPUImpl::fused_first_pass() = on != 0;
+    return prev;
+}
+
 }  // extern "C"
 
 extern "C" {

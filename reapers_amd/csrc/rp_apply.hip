@@ -433,29 +433,22 @@ __device__ __forceinline__ void flush_partials(A (&accr)[RP], A (&acci)[RP], flo
     }
 }
 
+// The body of the tiled apply for the output tile of this workgroup: every
+// group's input tile through the LDS pipeline, then the tile written once.
+// wv receives the values written (the rounded outputs), for an epilogue.
 template <typename R, typename A, int TB, int NT, int D, bool GA>
-__global__ void __launch_bounds__(NT, (apply_waves_per_eu<R, TB, NT, D>()))
-k_apply_tiled(const typename CxA<R>::T *__restrict__ in, typename CxA<R>::T *__restrict__ out,
-              const GroupRuns *__restrict__ groups, int ngroups, const SubRec<A> *__restrict__ subs,
-              const TermRec<A> *__restrict__ terms, int accumulate)
+__device__ __forceinline__ void apply_tile(const typename CxA<R>::T *__restrict__ in,
+                                           typename CxA<R>::T *__restrict__ out,
+                                           const GroupRuns *__restrict__ groups, int ngroups,
+                                           const SubRec<A> *__restrict__ subs,
+                                           const TermRec<A> *__restrict__ terms, int accumulate,
+                                           typename CxA<R>::T *tile0, typename CxA<R>::T *tile1,
+                                           typename CxA<R>::T *tile2, typename CxA<R>::T *tile3,
+                                           uint32_t t0, typename CxA<R>::T (&wv)[(1 << TB) / NT])
 {
     using T = typename CxA<R>::T;
     constexpr int TILE = 1 << TB;
     constexpr int RP = TILE / NT;          // amplitudes per thread
-    constexpr int LB = NT == 1024 ? 10 : NT == 512 ? 9 : NT == 256 ? 8 : 7;
-    static_assert((1 << LB) == NT, "NT must be a power of two in [128, 1024]");
-    static_assert(D >= 2 && D <= 4, "2 to 4 LDS tiles");
-    // D separate LDS tiles (the unused ones have one element)
-    __shared__ T tile0[TILE];
-    __shared__ T tile1[TILE];
-    __shared__ T tile2[D > 2 ? TILE : 1];
-    __shared__ T tile3[D > 3 ? TILE : 1];
-
-    // XCD-aware numbering: consecutive workgroup ids go to different XCDs,
-    // so give each XCD a contiguous range of tiles
-    const uint32_t nb = gridDim.x, b = blockIdx.x;
-    const uint32_t tid_tile = (nb & 7u) == 0 ? (b & 7u) * (nb >> 3) + (b >> 3) : b;
-    const uint32_t t0 = tid_tile << TB;
     const uint32_t tid = threadIdx.x;
 
     A accr[RP], acci[RP];
@@ -509,6 +502,124 @@ k_apply_tiled(const typename CxA<R>::T *__restrict__ in, typename CxA<R>::T *__r
             o.y = (R)acci[r];
         }
         out[m] = o;
+        wv[r] = o;
+    }
+}
+
+// XCD-aware numbering: consecutive workgroup ids go to different XCDs, so
+// give each XCD a contiguous range of tiles
+__device__ __forceinline__ uint32_t xcd_tile_index()
+{
+    const uint32_t nb = gridDim.x, b = blockIdx.x;
+    return (nb & 7u) == 0 ? (b & 7u) * (nb >> 3) + (b >> 3) : b;
+}
+
+template <typename R, typename A, int TB, int NT, int D, bool GA>
+__global__ void __launch_bounds__(NT, (apply_waves_per_eu<R, TB, NT, D>()))
+k_apply_tiled(const typename CxA<R>::T *__restrict__ in, typename CxA<R>::T *__restrict__ out,
+              const GroupRuns *__restrict__ groups, int ngroups, const SubRec<A> *__restrict__ subs,
+              const TermRec<A> *__restrict__ terms, int accumulate)
+{
+    using T = typename CxA<R>::T;
+    constexpr int TILE = 1 << TB;
+    static_assert(NT == 1024 || NT == 512 || NT == 256 || NT == 128,
+                  "NT must be a power of two in [128, 1024]");
+    static_assert(D >= 2 && D <= 4, "2 to 4 LDS tiles");
+    // D separate LDS tiles (the unused ones have one element)
+    __shared__ T tile0[TILE];
+    __shared__ T tile1[TILE];
+    __shared__ T tile2[D > 2 ? TILE : 1];
+    __shared__ T tile3[D > 3 ? TILE : 1];
+    T wv[TILE / NT];
+    apply_tile<R, A, TB, NT, D, GA>(in, out, groups, ngroups, subs, terms, accumulate, tile0, tile1,
+                                    tile2, tile3, xcd_tile_index() << TB, wv);
+}
+
+// ---------------------------------------------------------------------------
+// The apply with the first Gram-Schmidt pass's coefficients formed in its
+// epilogue: after writing its output tile w, the workgroup reads the same
+// rows of V_0..V_{k-1} and writes its FP64 partials of <V_q|w> (columns
+// 2q, 2q+1) and of ||w||^2 (column 2 kMaxSweep) to row blockIdx.x of part.
+// The w values are the rounded ones just written, so the coefficients are
+// those of the stored vector.
+// ---------------------------------------------------------------------------
+constexpr int kMaxSweep = RP_MAX_SWEEP_VECS;
+constexpr int kEpiChunk = 8;                        // vectors per block reduction
+struct VecPtrs64 { const void *p[kMaxSweep]; };
+
+// sum NV per-thread doubles over the NT-thread workgroup into sh[(NT/64)*NV + q]
+template <int NT, int NV>
+__device__ __forceinline__ void wg_reduce(double (&v)[NV], double *sh)
+{
+    constexpr int W = NT / 64;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+        double x = v[q];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+        v[q] = x;
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; q++) sh[wid * NV + q] = v[q];
+    }
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        double s = 0;
+        for (int w = 0; w < W; w++) s += sh[w * NV + threadIdx.x];
+        sh[W * NV + threadIdx.x] = s;
+    }
+    __syncthreads();
+}
+
+template <typename R, typename A, int TB, int NT, int D, bool GA>
+__global__ void __launch_bounds__(NT, (apply_waves_per_eu<R, TB, NT, D>()))
+k_apply_tiled_gs(const typename CxA<R>::T *__restrict__ in, typename CxA<R>::T *__restrict__ out,
+                 const GroupRuns *__restrict__ groups, int ngroups,
+                 const SubRec<A> *__restrict__ subs, const TermRec<A> *__restrict__ terms,
+                 VecPtrs64 V, int k, double *__restrict__ part, int width)
+{
+    using T = typename CxA<R>::T;
+    constexpr int TILE = 1 << TB;
+    constexpr int RP = TILE / NT;
+    constexpr int NV = 2 * kEpiChunk + 1;
+    __shared__ T tile0[TILE];
+    __shared__ T tile1[TILE];
+    __shared__ T tile2[D > 2 ? TILE : 1];
+    __shared__ T tile3[D > 3 ? TILE : 1];
+    __shared__ double sh[(NT / 64 + 1) * NV];
+    T wv[RP];
+    const uint32_t t0 = xcd_tile_index() << TB;
+    apply_tile<R, A, TB, NT, D, GA>(in, out, groups, ngroups, subs, terms, 0, tile0, tile1, tile2,
+                                    tile3, t0, wv);
+    const uint32_t tid = threadIdx.x;
+    double wn2 = 0;
+#pragma unroll
+    for (int r = 0; r < RP; r++) wn2 += (double)wv[r].x * wv[r].x + (double)wv[r].y * wv[r].y;
+    double *row = part + (size_t)blockIdx.x * width;
+    for (int q0 = 0; q0 == 0 || q0 < k; q0 += kEpiChunk) {
+        double acc[NV];
+#pragma unroll
+        for (int q = 0; q < NV; q++) acc[q] = 0;
+#pragma unroll
+        for (int q = 0; q < kEpiChunk; q++) {
+            if (q0 + q < k) {
+                const T *vq = (const T *)V.p[q0 + q];
+#pragma unroll
+                for (int r = 0; r < RP; r++) {
+                    const T x = vq[(uint64_t)t0 + r * NT + tid];
+                    const double xr = x.x, xi = x.y, yr = wv[r].x, yi = wv[r].y;
+                    acc[2 * q] += xr * yr + xi * yi;
+                    acc[2 * q + 1] += xr * yi - xi * yr;
+                }
+            }
+        }
+        if (q0 == 0) acc[2 * kEpiChunk] = wn2;
+        wg_reduce<NT, NV>(acc, sh);
+        constexpr int W = NT / 64;
+        if ((int)tid < 2 * kEpiChunk && q0 + (int)tid / 2 < k) row[2 * q0 + tid] = sh[W * NV + tid];
+        if (q0 == 0 && tid == 2 * kEpiChunk) row[2 * kMaxSweep] = sh[W * NV + tid];
     }
 }
 
@@ -700,6 +811,38 @@ static int apply_impl(rp_terms *h, void *res, const void *vec, int accumulate)
     }
     return set_error(RP_ERR_RUNTIME, "rp_apply_ops: unsupported tiling b=%d threads=%d depth=%d",
                      h->tile_bits, h->threads, h->depth);
+}
+
+// rp_apply_gs's kernel: the tiled apply with the Gram-Schmidt dots in its
+// epilogue, for the tilings it is instantiated for (complex<float> states,
+// 2^12 tiles, 512 threads, 2 LDS tiles, FP64 or group accumulation) and at
+// most max_rows workgroups; *rows = the partial rows written, 0 when the
+// handle's layout has no such kernel (the caller then applies and sweeps
+// separately)
+int apply_dots_launch(rp_terms *h, void *res, const void *vec, const void *const *V, int k,
+                      double *part, int width, uint64_t max_rows, uint64_t *rows)
+{
+    using T = float2;
+    *rows = 0;
+    if (h->prec != RP_C64 || !h->acc64 || h->nterms == 0 || h->tile_bits != 12 ||
+        h->threads != 512 || h->depth != 2 || k < 0 || k > kMaxSweep)
+        return RP_OK;
+    const uint64_t grid = (1ull << h->len) >> 12;
+    if (grid > max_rows) return RP_OK;
+    VecPtrs64 P{};
+    for (int q = 0; q < k; q++) P.p[q] = V[q];
+    if (h->group_acc)
+        RP_LAUNCH(RP_KERNEL_APPLY, (k_apply_tiled_gs<float, double, 12, 512, 2, true>),
+                  dim3((unsigned)grid), dim3(512), 0, (const T *)vec, (T *)res,
+                  (const GroupRuns *)h->d_groups, (int)h->ngroups, (const SubRec<double> *)h->d_subs,
+                  (const TermRec<double> *)h->d_terms, P, k, part, width);
+    else
+        RP_LAUNCH(RP_KERNEL_APPLY, (k_apply_tiled_gs<float, double, 12, 512, 2, false>),
+                  dim3((unsigned)grid), dim3(512), 0, (const T *)vec, (T *)res,
+                  (const GroupRuns *)h->d_groups, (int)h->ngroups, (const SubRec<double> *)h->d_subs,
+                  (const TermRec<double> *)h->d_terms, P, k, part, width);
+    *rows = grid;
+    return RP_OK;
 }
 
 }  // namespace rp

@@ -372,24 +372,44 @@ k_batch_dots(const void *const *__restrict__ Vt, int k, const typename CxA<R>::T
     }
 }
 
-// h[rhs][i] = sum over the problem's chunks (in order) of part; the column
-// of the Hessenberg matrix accumulates h over the passes
+// Sum of the values src[c * stride] over chunks c in [c0, c0 + nc) by the
+// block's kRedT threads: thread t takes chunks c0 + t, c0 + t + kRedT, ...
+// in order, then a fixed pairwise tree over the threads (deterministic for a
+// given chunk count).  The result is valid in thread 0.
+__device__ inline double block_chunk_sum(const double *__restrict__ src, size_t stride,
+                                         uint32_t c0, uint32_t nc, double *red)
+{
+    double a = 0;
+    for (uint32_t c = threadIdx.x; c < nc; c += kRedT) a += src[(size_t)(c0 + c) * stride];
+    red[threadIdx.x] = a;
+    __syncthreads();
+#pragma unroll
+    for (int w = kRedT / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    return red[0];
+}
+
+// h[rhs][i] = sum over the problem's chunks of part; the column of the
+// Hessenberg matrix accumulates h over the passes.  Block (p, e) reduces
+// element e = i * ns + s of problem p.
 __global__ void __launch_bounds__(kRedT)
 k_batch_reduce_dots(const double *__restrict__ part, int k, const ProbDesc *__restrict__ probs,
                     const int *__restrict__ active, int ns, double *__restrict__ h, int hstride,
                     double *__restrict__ H, int j, int kmax, int first_pass)
 {
-    const int p = blockIdx.x;
-    if (!active[p]) return;
+    __shared__ double red[kRedT];
+    const int p = blockIdx.x, e = blockIdx.y;
+    if (!active[p] || e >= k * ns) return;
     const ProbDesc P = probs[p];
-    for (int e = threadIdx.x; e < k * ns; e += blockDim.x) {
-        const int i = e / ns, s = e % ns;
-        double a = 0, b = 0;
-        for (uint32_t c = P.chunk0; c < P.chunk0 + P.nchunks; c++) {
-            const double *q = part + (size_t)c * (2 * (size_t)ns * 65) + 2 * ((size_t)i * ns + s);
-            a += q[0];
-            b += q[1];
-        }
+    const int i = e / ns, s = e % ns;
+    const size_t stride = 2 * (size_t)ns * 65;
+    const double *q = part + 2 * ((size_t)i * ns + s);
+    const double a = block_chunk_sum(q, stride, P.chunk0, P.nchunks, red);
+    __syncthreads();
+    const double b = block_chunk_sum(q + 1, stride, P.chunk0, P.nchunks, red);
+    if (threadIdx.x == 0) {
         const size_t rhs = (size_t)p * ns + s;
         h[rhs * hstride + 2 * i] = a;
         h[rhs * hstride + 2 * i + 1] = b;
@@ -472,19 +492,21 @@ k_batch_norm_part(const typename CxA<R>::T *__restrict__ v, const ProbDesc *__re
 // (0 if nrm == 0); optionally H(j+1, j) = nrm and nrm_out[rhs] = nrm
 // im_only: write the norm to Im H(j+1, j) only (the norm after the first
 // Gram-Schmidt pass, kept for the breakdown test) and leave inv alone
-__global__ void k_batch_norm_final(const double *__restrict__ npart,
-                                   const ProbDesc *__restrict__ probs,
-                                   const int *__restrict__ active, int nprob, int ns,
-                                   double *__restrict__ inv, double *__restrict__ nrm_out,
-                                   double *__restrict__ H, int j, int kmax, int im_only)
+// ||v|| of every rhs from the chunk partials npart[c][s]: block rhs
+__global__ void __launch_bounds__(kRedT)
+k_batch_norm_final(const double *__restrict__ npart, const ProbDesc *__restrict__ probs,
+                   const int *__restrict__ active, int nprob, int ns,
+                   double *__restrict__ inv, double *__restrict__ nrm_out,
+                   double *__restrict__ H, int j, int kmax, int im_only)
 {
-    const int rhs = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ double red[kRedT];
+    const int rhs = blockIdx.x;
     if (rhs >= nprob * ns) return;
     const int p = rhs / ns, s = rhs % ns;
     if (!active[p]) return;
     const ProbDesc P = probs[p];
-    double a = 0;
-    for (uint32_t c = P.chunk0; c < P.chunk0 + P.nchunks; c++) a += npart[(size_t)c * ns + s];
+    const double a = block_chunk_sum(npart + s, (size_t)ns, P.chunk0, P.nchunks, red);
+    if (threadIdx.x != 0) return;
     const double n = sqrt(a);
     double *Hij = H ? H + (size_t)rhs * (2 * (kmax + 1) * kmax) + 2 * ((size_t)j * (kmax + 1) + j + 1)
                     : nullptr;
@@ -968,7 +990,7 @@ int rp_batch_normalize(rp_batch_t b, int i, double *nrm_out)
         RP_LAUNCH(RP_KERNEL_GS, k_batch_norm_part<double>, gc, dim3(kRedT), 0,
                   (const double2 *)b->V[i], P, cp, act, b->ns, b->chunk_bits, (double *)b->d_npart);
     }
-    RP_LAUNCH(RP_KERNEL_GS, k_batch_norm_final, dim3((nr + 255) / 256), dim3(256), 0,
+    RP_LAUNCH(RP_KERNEL_GS, k_batch_norm_final, dim3(nr), dim3(kRedT), 0,
               (const double *)b->d_npart, P, act, b->nprob, b->ns, (double *)b->d_inv, nrm_out,
               (double *)nullptr, 0, b->kmax, 0);
     if (b->prec == RP_C64) {
@@ -997,7 +1019,7 @@ int rp_batch_arnoldi_step(rp_batch_t b, int j, double *H)
     const uint32_t *cp = (const uint32_t *)b->d_chunk_prob;
     const int *act = (const int *)b->d_active;
     const void *const *Vt = (const void *const *)b->d_V;
-    const dim3 gc((unsigned)b->total_chunks), gp((unsigned)b->nprob);
+    const dim3 gc((unsigned)b->total_chunks);
     const int k = j + 1, hstride = 65 * 2, nr = b->nprob * b->ns;
     for (int pass = 0; pass < 2; pass++) {
         if (b->prec == RP_C64) {
@@ -1009,7 +1031,8 @@ int rp_batch_arnoldi_step(rp_batch_t b, int j, double *H)
                       (const double2 *)b->V[j + 1], P, cp, act, b->ns, b->chunk_bits,
                       (double *)b->d_part);
         }
-        RP_LAUNCH(RP_KERNEL_GS, k_batch_reduce_dots, gp, dim3(kRedT), 0,
+        RP_LAUNCH(RP_KERNEL_GS, k_batch_reduce_dots, dim3((unsigned)b->nprob, (unsigned)(k * b->ns)),
+                  dim3(kRedT), 0,
                   (const double *)b->d_part, k, P, act, b->ns, (double *)b->d_h, hstride, H, j,
                   b->kmax, pass == 0 ? 1 : 0);
         if (b->prec == RP_C64) {
@@ -1023,7 +1046,7 @@ int rp_batch_arnoldi_step(rp_batch_t b, int j, double *H)
         }
         // after the first pass: its norm to Im H(j+1, j); after the second:
         // the norm to Re H(j+1, j) and its inverse for the normalization
-        RP_LAUNCH(RP_KERNEL_GS, k_batch_norm_final, dim3((nr + 255) / 256), dim3(256), 0,
+        RP_LAUNCH(RP_KERNEL_GS, k_batch_norm_final, dim3(nr), dim3(kRedT), 0,
                   (const double *)b->d_npart, P, act, b->nprob, b->ns, (double *)b->d_inv,
                   (double *)nullptr, H, j, b->kmax, pass == 0 ? 1 : 0);
     }

@@ -36,6 +36,10 @@ struct Context {
     // pinned host buffer for scalars copied back to the host
     double *pinned = nullptr;
     size_t pinned_doubles = 0;
+    // partial rows of the apply's Gram-Schmidt epilogue (rp_apply_gs),
+    // allocated on first use
+    void *epi_work = nullptr;
+    size_t epi_bytes = 0;
     // profiling
     bool prof = false;
     int prof_mask = 7;                // kernel classes bracketed (bit = class)
@@ -61,6 +65,11 @@ size_t pooled_bytes();
 // cannot fall back on the pool.
 void ensure_device_headroom(size_t bytes);
 int check_init();
+// the apply with the Gram-Schmidt dots in its epilogue (rp_apply.hip); rows
+// = partial rows written to part (row stride width doubles), 0 if the term
+// handle's layout has no such kernel
+int apply_dots_launch(rp_terms_t h, void *res, const void *vec, const void *const *V, int k,
+                      double *part, int width, uint64_t max_rows, uint64_t *rows);
 
 // launch bracketing for the optional per-class timing
 void prof_begin(int cls, hipEvent_t *start, hipEvent_t *stop);

@@ -138,9 +138,12 @@ int rp_shutdown(void)
     c.event_pool.clear();
     (void)release_pool();
     if (c.work) (void)hipFree(c.work);
+    if (c.epi_work) (void)hipFree(c.epi_work);
     if (c.pinned) (void)hipHostFree(c.pinned);
     if (c.stream) (void)hipStreamDestroy(c.stream);
     c.work = nullptr;
+    c.epi_work = nullptr;
+    c.epi_bytes = 0;
     c.pinned = nullptr;
     c.stream = nullptr;
     c.initialized = false;

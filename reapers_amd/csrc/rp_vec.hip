@@ -12,6 +12,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 namespace rp {
 
@@ -365,11 +366,12 @@ static inline int red_blocks(uint64_t n)
     return grid_for(n, kRedThreads, kRedBlocks);
 }
 
-// copy `cnt` doubles from the device result area to the host (synchronizes)
-static int fetch_results(double *dst, int cnt)
+// copy `cnt` doubles from a device result area (default: the workspace's)
+// to the host (synchronizes)
+static int fetch_results(double *dst, int cnt, const double *src = nullptr)
 {
     Context &c = ctx();
-    RP_CHECK_HIP(hipMemcpyAsync(c.pinned, ws_result(), sizeof(double) * cnt,
+    RP_CHECK_HIP(hipMemcpyAsync(c.pinned, src ? src : ws_result(), sizeof(double) * cnt,
                                 hipMemcpyDeviceToHost, c.stream));
     RP_CHECK_HIP(hipStreamSynchronize(c.stream));
     prof_collect();
@@ -390,14 +392,15 @@ static int dot_impl(const void *a, const void *b, uint64_t n, double *out2)
 }
 
 template <typename R>
-static int nrm2_launch(const void *a, uint64_t n, double *dst_dev)
+static int nrm2_launch(const void *a, uint64_t n, double *dst_dev, double *part = nullptr)
 {
     using T = typename Cx<R>::T;
     int nb = red_blocks(n);
+    if (!part) part = ws_partial();
     RP_LAUNCH(RP_KERNEL_OTHER, k_nrm2_partial<R>, dim3(nb), dim3(kRedThreads), 0,
-              (const T *)a, n, ws_partial());
+              (const T *)a, n, part);
     RP_LAUNCH(RP_KERNEL_OTHER, k_reduce_cols, dim3(1), dim3(kRedThreads), 0,
-              (const double *)ws_partial(), nb, 1, dst_dev);
+              (const double *)part, nb, 1, dst_dev);
     return RP_OK;
 }
 
@@ -416,17 +419,30 @@ static int nrm2_launch(const void *a, uint64_t n, double *dst_dev)
 
 // One Gram-Schmidt pass; res layout: [0, 2k) h, [2 kMaxVecs] ||w_out||^2,
 // [2 kMaxVecs + 1] ||w_in||^2, [2 kMaxVecs + 2] fused beta (0: not fused)
+// pre / pre_rows: the dot partials of w already formed by the apply's
+// epilogue (rows of width doubles, ||w||^2 in column 2 kMaxVecs); the pass
+// then starts from their reduction
 template <typename R>
 static int gs_pass_impl(const void *const *V, int k, void *w, uint64_t n, double *h,
-                        double *norm, double fuse_min, int *normalized)
+                        double *norm, double fuse_min, int *normalized, double *work = nullptr,
+                        const double *pre = nullptr, uint64_t pre_rows = 0)
 {
     using T = typename Cx<R>::T;
     const int nb = red_blocks(n);
     const int width = 2 * kMaxVecs + 1;
-    double *res = ws_result();
-    double *npart = ws_partial() + (size_t)kRedBlocks * (2 * kMaxVecs);   // norm partials
+    // the workspace: the library's, or the caller-chosen one of the same layout
+    double *const part = work ? work : ws_partial();
+    double *res = part + (size_t)kRedBlocks * (2 * kMaxVecs + 1);
+    double *npart = part + (size_t)kRedBlocks * (2 * kMaxVecs);   // norm partials
     const bool fuse = fuse_min > 0 && k > 0;
-    if (k > 0) {
+    if (pre) {
+        if (k > 0)
+            RP_LAUNCH(RP_KERNEL_GS, k_reduce_cols, dim3(2 * k), dim3(kRedThreads), 0, pre,
+                      (int)pre_rows, width, res);
+        RP_LAUNCH(RP_KERNEL_GS, k_reduce_cols, dim3(1), dim3(kRedThreads), 0,
+                  pre + 2 * kMaxVecs, (int)pre_rows, width,
+                  res + 2 * kMaxVecs + (k > 0 ? 1 : 0));
+    } else if (k > 0) {
         for (int q0 = 0; q0 < k; q0 += kDotChunk) {
             const int kc = k - q0 < kDotChunk ? k - q0 : kDotChunk;
             VecPtrs16 P{};
@@ -434,20 +450,22 @@ static int gs_pass_impl(const void *const *V, int k, void *w, uint64_t n, double
             const int wn = fuse && q0 == 0;
 #define RP_DOTS(KC)                                                               \
     RP_LAUNCH(RP_KERNEL_GS, (k_dots<R, KC>), dim3(nb), dim3(kRedThreads), 0, P, (const T *)w, n, \
-              ws_partial(), width, q0, wn)
+              part, width, q0, wn)
             RP_KC_SWITCH(kc, RP_DOTS)
 #undef RP_DOTS
         }
         RP_LAUNCH(RP_KERNEL_GS, k_reduce_cols, dim3(2 * k), dim3(kRedThreads), 0,
-                  (const double *)ws_partial(), nb, width, res);
+                  (const double *)part, nb, width, res);
         if (fuse)
             RP_LAUNCH(RP_KERNEL_GS, k_reduce_cols, dim3(1), dim3(kRedThreads), 0,
-                      (const double *)ws_partial() + 2 * kMaxVecs, nb, width,
+                      (const double *)part + 2 * kMaxVecs, nb, width,
                       res + 2 * kMaxVecs + 1);
     }
     if (k == 0) {
-        int st = nrm2_launch<R>(w, n, res + 2 * kMaxVecs);
-        if (st) return st;
+        if (!pre) {
+            int st = nrm2_launch<R>(w, n, res + 2 * kMaxVecs, part);
+            if (st) return st;
+        }
     } else {
         for (int q0 = 0; q0 < k; q0 += kUpdChunk) {
             const int kc = k - q0 < kUpdChunk ? k - q0 : kUpdChunk;
@@ -465,7 +483,7 @@ static int gs_pass_impl(const void *const *V, int k, void *w, uint64_t n, double
                   (const double *)npart, nb, 1, res + 2 * kMaxVecs);
     }
     double tmp[2 * kMaxVecs + 3];
-    int st = fetch_results(tmp, 2 * kMaxVecs + 3);
+    int st = fetch_results(tmp, 2 * kMaxVecs + 3, res);
     if (st) return st;
     for (int q = 0; q < 2 * k; q++) h[q] = tmp[q];
     const double beta = fuse ? tmp[2 * kMaxVecs + 2] : 0.0;
@@ -701,6 +719,60 @@ int rp_gs_pass_fused(int prec, const void *const *V, int k, void *w, uint64_t n,
     }
     return prec == RP_C64 ? gs_pass_impl<float>(V, k, w, n, h, norm, fuse_min, normalized)
                           : gs_pass_impl<double>(V, k, w, n, h, norm, fuse_min, normalized);
+}
+
+int rp_apply_gs(rp_terms_t hd, void *res, const void *vec, const void *const *V, int k,
+                double *h, double *norm, double fuse_min, int *normalized)
+{
+    int st = check_init();
+    if (st) return st;
+    RP_REQUIRE(hd, "null term handle");
+    RP_REQUIRE(res && vec && res != vec, "res and vec must be different non-null buffers");
+    RP_REQUIRE(k >= 0 && k <= kMaxVecs, "k must be in [0, 64]");
+    RP_REQUIRE(norm && normalized && (k == 0 || (h && V)), "null argument");
+    for (int q = 0; q < k; q++) {
+        RP_REQUIRE(V[q], "null basis vector");
+        RP_REQUIRE(V[q] != res, "res must not be one of the basis vectors");
+    }
+    rp_terms_info_t info;
+    if ((st = rp_terms_info(hd, &info))) return st;
+    const uint64_t n = 1ull << info.len;
+    *normalized = 0;
+    Context &c = ctx();
+    const int width = 2 * kMaxVecs + 1;
+    // partial rows for up to 2^16 workgroups (2^28 amplitudes at 2^12 per tile)
+    const uint64_t max_rows = 1ull << 16;
+    const uint64_t want = (n >> 12) < max_rows ? (n >> 12) : max_rows;
+    const size_t need = (size_t)want * width * sizeof(double);
+    if (want && c.epi_bytes < need) {
+        if (c.epi_work) {
+            RP_CHECK_HIP(hipStreamSynchronize(c.stream));
+            (void)hipFree(c.epi_work);
+        }
+        c.epi_work = nullptr;
+        c.epi_bytes = 0;
+        if (dev_malloc(&c.epi_work, need) != hipSuccess) {
+            (void)hipGetLastError();
+            c.epi_work = nullptr;
+        } else {
+            c.epi_bytes = need;
+        }
+    }
+    uint64_t rows = 0;
+    if (c.epi_work) {
+        st = apply_dots_launch(hd, res, vec, V, k, (double *)c.epi_work, width,
+                               c.epi_bytes / (width * sizeof(double)), &rows);
+        if (st) return st;
+    }
+    if (rows == 0) {
+        if ((st = rp_apply_ops(hd, res, vec, 0))) return st;
+        return rp_gs_pass_fused(info.prec, V, k, res, n, h, norm, fuse_min, normalized);
+    }
+    return info.prec == RP_C64
+               ? gs_pass_impl<float>(V, k, res, n, h, norm, fuse_min, normalized, nullptr,
+                                     (const double *)c.epi_work, rows)
+               : gs_pass_impl<double>(V, k, res, n, h, norm, fuse_min, normalized, nullptr,
+                                      (const double *)c.epi_work, rows);
 }
 
 int rp_lincomb(int prec, void *y, const void *const *V, int k, const double *c, uint64_t n)

@@ -36,9 +36,12 @@ class Krylov {
     // iterated fused Gram-Schmidt of buf(j+1) against buf(0..j).  A pass
     // that ends the iteration (its norm at least eta times the previous one)
     // may normalize buf(j+1) in the same sweep; normalized reports it.
-    template<RealScalar FpType>
+    // first(V, r, fuse_min, normalized) makes the first pass (by default the
+    // plain sweep; the Arnoldi step passes the apply fused with it).
+    template<RealScalar FpType, typename FirstPass>
     static bool orthogonalize(State<FpType, Impl> &s, dense::CMat &H, int j, double &beta,
-                              Reorth mode, KrylovStats &st, bool &normalized) {
+                              Reorth mode, KrylovStats &st, bool &normalized,
+                              FirstPass &&first) {
         const double eta = 1 / std::sqrt(2.0);
         const size_t dim = s.dim();
         std::vector<const typename Impl::template VecType<FpType> *> V(j + 1);
@@ -55,7 +58,8 @@ class Krylov {
             double fuse_min = 0;
             if (mode == Reorth::Always) fuse_min = niter == max_passes ? 1e-300 : 0;
             else fuse_min = eta * norm;
-            double nrm = Impl::gs_pass(dim, V, s.buf(j + 1), r, fuse_min, normalized);
+            double nrm = niter == 1 ? first(V, r, fuse_min, normalized)
+                                    : Impl::gs_pass(dim, V, s.buf(j + 1), r, fuse_min, normalized);
             for (int i = 0; i <= j; i++) h[i] += r[i];
             old_norm = norm;
             norm = nrm;
@@ -75,10 +79,38 @@ class Krylov {
         const size_t dim = s.dim();
         const auto len = s.spin_chain_length();
         for (int j = j0; j < m; j++) {
-            Impl::apply_ops_overwrite(len, s.buf(j + 1), A, s.buf(j));
+            bool normalized, ok;
+            using VecList = std::vector<const typename Impl::template VecType<FpType> *>;
+            bool fused = false;
+            if constexpr (requires(VecList &V, std::vector<cd> &r, bool &nz) {
+                              Impl::apply_gs(len, s.buf(j + 1), A, s.buf(j), V, r, 0.0, nz);
+                              Impl::fused_first_pass();
+                          })
+                fused = Impl::fused_first_pass();
+            if (fused) {
+                // the apply and the first Gram-Schmidt pass in one sweep
+                ok = orthogonalize(s, H, j, beta, mode, st, normalized,
+                                   [&](const VecList &V, std::vector<cd> &r, double fuse_min,
+                                       bool &nz) {
+                                       if constexpr (requires {
+                                                         Impl::apply_gs(len, s.buf(j + 1), A,
+                                                                        s.buf(j), V, r, fuse_min,
+                                                                        nz);
+                                                     })
+                                           return Impl::apply_gs(len, s.buf(j + 1), A, s.buf(j),
+                                                                 V, r, fuse_min, nz);
+                                       else
+                                           return 0.0;
+                                   });
+            } else {
+                Impl::apply_ops_overwrite(len, s.buf(j + 1), A, s.buf(j));
+                ok = orthogonalize(s, H, j, beta, mode, st, normalized,
+                                   [&](const VecList &V, std::vector<cd> &r, double fuse_min,
+                                       bool &nz) {
+                                       return Impl::gs_pass(dim, V, s.buf(j + 1), r, fuse_min, nz);
+                                   });
+            }
             st.matvecs++;
-            bool normalized;
-            bool ok = orthogonalize(s, H, j, beta, mode, st, normalized);
             if (!ok) {
                 m = j + 1;
                 return false;

@@ -419,6 +419,39 @@ public:
         return nrm;
     }
 
+    // Whether Krylov's Arnoldi steps use apply_gs for the apply and their
+    // first Gram-Schmidt pass.  Off by default: at 24 qubits the epilogue's
+    // reads of the basis cost the apply more (65.9 -> 77.6 ms per N=48
+    // evolution) than the separate dot sweep it saves (21.0 -> 11.8 ms).
+    static bool &fused_first_pass() {
+        static bool on = false;
+        return on;
+    }
+
+    // w = ops * v fused with the first Gram-Schmidt pass of w against V
+    // (rp_apply_gs: the pass's coefficients formed by the apply kernel as it
+    // writes w); returns ||w|| after the pass, or beta when normalized
+    template<RealScalar FpType>
+    static double apply_gs(typename SpinOp<FpType>::IndexType len, BufType<FpType> w,
+                           const DevSumOps<FpType> &ops, ConstBufType<FpType> v,
+                           const std::vector<const DeviceVec<FpType> *> &V,
+                           std::vector<std::complex<double>> &h, double fuse_min,
+                           bool &normalized) {
+        if (use_dense(len, ops)) {
+            dense_apply_impl(len, w, ops, v, false);
+            return gs_pass((size_t)1 << len, V, w, h, fuse_min, normalized);
+        }
+        std::vector<const void *> p(V.size());
+        for (size_t i = 0; i < V.size(); i++) p[i] = V[i]->ptr;
+        h.resize(V.size());
+        double nrm;
+        int nz = 0;
+        RP_CALL(rp_apply_gs(ops.device_terms(len), w.ptr, v.ptr, p.data(), (int)V.size(),
+                            (double *)h.data(), &nrm, fuse_min, &nz));
+        normalized = nz != 0;
+        return nrm;
+    }
+
     // res = mat * vec for a dim x dim device matrix (mat_mul,
     // inc/gpuimpl.h:1009-1022); res and vec must differ
     template<RealScalar FpType, typename Mat>

@@ -24,6 +24,8 @@
 #include <iostream>
 #include <map>
 #include <memory>
+#include <numeric>
+#include <optional>
 #include <random>
 #include <set>
 #include <sstream>
