@@ -205,6 +205,17 @@ int rp_gs_pass(int prec, const void *const *V, int k, void *w, uint64_t n,
  * Arnoldi step (inc/algo.h:170). */
 int rp_gs_pass_fused(int prec, const void *const *V, int k, void *w, uint64_t n, double *h,
                      double *norm, double fuse_min, int *normalized);
+/* rp_gs_pass_fused that allocates nothing, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid it); no counterpart in the reference.  Same arguments and
+ * results as rp_gs_pass_fused; its reduction partials and results live in a
+ * fixed 1.06 MB device array of the library (one per GPU, taken at rp_init)
+ * instead of the library workspace, which limits k to 64 (larger k is
+ * rejected with RP_ERR_INVALID_ARGUMENT; n is unlimited).  Every call on a
+ * GPU shares that GPU's array and leaves its last contents there: calls that
+ * overlap on one GPU (from several host threads or streams) race. */
+int rp_gs_pass_fused_noalloc(int prec, const void *const *V, int k, void *w, uint64_t n,
+                             double *h, double *norm, double fuse_min, int *normalized);
 /* y += sum_i c[i] V[i] for k vectors, c given as 2k doubles (re, im) */
 int rp_lincomb(int prec, void *y, const void *const *V, int k, const double *c, uint64_t n);
 

@@ -98,6 +98,8 @@ def hip():
             "rp_gs_pass": [_c.c_int, _vp, _c.c_int, _vp, _u64, _vp, _c.POINTER(_dbl)],
             "rp_gs_pass_fused": [_c.c_int, _vp, _c.c_int, _vp, _u64, _vp, _c.POINTER(_dbl), _dbl,
                                  _c.POINTER(_c.c_int)],
+            "rp_gs_pass_fused_noalloc": [_c.c_int, _vp, _c.c_int, _vp, _u64, _vp,
+                                         _c.POINTER(_dbl), _dbl, _c.POINTER(_c.c_int)],
             "rp_lincomb": [_c.c_int, _vp, _vp, _c.c_int, _vp, _u64],
             "rp_prof_enable": [_c.c_int], "rp_prof_reset": [],
             "rp_prof_query": [_c.c_int, _c.POINTER(_u64), _c.POINTER(_dbl)],

@@ -65,6 +65,9 @@ size_t pooled_bytes();
 // cannot fall back on the pool.
 void ensure_device_headroom(size_t bytes);
 int check_init();
+// take the address of rp_vec.hip's file-scope scratch array on the current
+// device (once per device, at rp_init)
+int vec_scratch_setup(int device);
 // the apply with the Gram-Schmidt dots in its epilogue (rp_apply.hip); rows
 // = partial rows written to part (row stride width doubles), 0 if the term
 // handle's layout has no such kernel

@@ -116,6 +116,8 @@ int rp_init(int device)
     c.pinned_doubles = 4096;
     RP_CHECK_HIP(hipHostMalloc((void **)&c.pinned, c.pinned_doubles * sizeof(double),
                                hipHostMallocDefault));
+    int st = vec_scratch_setup(device);
+    if (st) return st;
     c.initialized = true;
     return RP_OK;
 }

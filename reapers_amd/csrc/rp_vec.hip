@@ -361,6 +361,35 @@ static inline double *ws_result()
     return (double *)ctx().work + (size_t)kRedBlocks * (2 * kMaxVecs + 1);
 }
 
+// The Gram-Schmidt pass's workspace in a file-scope device array, for
+// rp_gs_pass_fused_noalloc: the partials and results of the library
+// workspace layout above, 1.06 MB.  Each device holds its own copy; the
+// host reaches it only through the address hipGetSymbolAddress returns for
+// the current device (taken once per device at rp_init).
+constexpr size_t kGsScratchDoubles = (size_t)kRedBlocks * (2 * kMaxVecs + 1) + 2 * kMaxVecs + 8;
+static __device__ __align__(256) double scratch[kGsScratchDoubles];
+static std::vector<double *> g_scratch_ptr;     // per device, null until set up
+
+int vec_scratch_setup(int device)
+{
+    if (device < 0) return set_error(RP_ERR_INVALID_ARGUMENT, "bad device %d", device);
+    if ((size_t)device >= g_scratch_ptr.size()) g_scratch_ptr.resize(device + 1, nullptr);
+    if (g_scratch_ptr[device]) return RP_OK;
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return set_error(RP_ERR_RUNTIME, "hipGetSymbolAddress(scratch) failed: %s",
+                         hipGetErrorString(e));
+    g_scratch_ptr[device] = (double *)p;
+    return RP_OK;
+}
+
+static double *vec_scratch(int device)
+{
+    if (device < 0 || (size_t)device >= g_scratch_ptr.size()) return nullptr;
+    return g_scratch_ptr[device];
+}
+
 static inline int red_blocks(uint64_t n)
 {
     return grid_for(n, kRedThreads, kRedBlocks);
@@ -719,6 +748,31 @@ int rp_gs_pass_fused(int prec, const void *const *V, int k, void *w, uint64_t n,
     }
     return prec == RP_C64 ? gs_pass_impl<float>(V, k, w, n, h, norm, fuse_min, normalized)
                           : gs_pass_impl<double>(V, k, w, n, h, norm, fuse_min, normalized);
+}
+
+int rp_gs_pass_fused_noalloc(int prec, const void *const *V, int k, void *w, uint64_t n,
+                             double *h, double *norm, double fuse_min, int *normalized)
+{
+    int st = check_init();
+    if (st) return st;
+    RP_REQUIRE(valid_prec(prec), "bad precision");
+    RP_REQUIRE(k >= 0 && k <= kMaxVecs,
+               "k must be in [0, 64]: the fixed scratch holds partials of 64 basis vectors");
+    RP_REQUIRE(norm && normalized && (k == 0 || (h && V)), "null argument");
+    RP_REQUIRE(w || n == 0, "null vector");
+    for (int q = 0; q < k; q++) RP_REQUIRE(V[q], "null basis vector");
+    double *work = vec_scratch(ctx().device);
+    if (!work)
+        return set_error(RP_ERR_RUNTIME, "rp_gs_pass_fused_noalloc: no scratch on device %d",
+                         ctx().device);
+    *normalized = 0;
+    if (n == 0) {
+        for (int q = 0; q < 2 * k; q++) h[q] = 0;
+        *norm = 0;
+        return RP_OK;
+    }
+    return prec == RP_C64 ? gs_pass_impl<float>(V, k, w, n, h, norm, fuse_min, normalized, work)
+                          : gs_pass_impl<double>(V, k, w, n, h, norm, fuse_min, normalized, work);
 }
 
 int rp_apply_gs(rp_terms_t hd, void *res, const void *vec, const void *const *V, int k,
