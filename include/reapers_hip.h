@@ -150,7 +150,14 @@ typedef struct {
     int      threads;       /* workgroup size of the tiled kernel */
     int      depth;         /* LDS tiles in its pipeline (input tiles in flight + 1);
                                for tile_bits 0 the number of term slices */
+    int      kernel;        /* RP_APPLY_SMALL, RP_APPLY_TILED or RP_APPLY_STREAM */
 } rp_terms_info_t;
+/* the apply kernel family a term handle launches: one thread per amplitude
+   (tile_bits 0); tiles with per-sub records; tiles with the compact sub
+   stream (complex<float>, group accumulation, two LDS tiles) */
+#define RP_APPLY_SMALL  0
+#define RP_APPLY_TILED  1
+#define RP_APPLY_STREAM 2
 int rp_terms_info(rp_terms_t h, rp_terms_info_t *info);
 /* Rebuild the layout for a given tiling of the tiled kernel (no counterpart
  * in the reference): tile bits / threads / pipeline depth must be one of the

@@ -23,6 +23,7 @@ RP_OK, RP_ERR_INVALID_ARGUMENT, RP_ERR_OUT_OF_MEMORY, RP_ERR_RUNTIME, RP_ERR_NO_
 RP_ROUND_NEAREST, RP_ROUND_TOWARD_ZERO = 0, 1
 RP_H2D, RP_D2H, RP_D2D = 0, 1, 2
 RP_KERNEL_APPLY, RP_KERNEL_GS, RP_KERNEL_OTHER = 0, 1, 2
+RP_APPLY_SMALL, RP_APPLY_TILED, RP_APPLY_STREAM = 0, 1, 2
 RP_TERMS_ACC_NATIVE, RP_TERMS_AUTOTUNE, RP_TERMS_ACC_GROUP = 1, 2, 4
 
 _c = ctypes
@@ -41,7 +42,8 @@ class RpError(RuntimeError):
 class TermsInfo(_c.Structure):
     _fields_ = [("prec", _c.c_int), ("len", _c.c_int), ("tile_bits", _c.c_int),
                 ("nterms", _u64), ("nflips", _u64), ("ngroups", _u64),
-                ("bytes_per_apply", _u64), ("threads", _c.c_int), ("depth", _c.c_int)]
+                ("bytes_per_apply", _u64), ("threads", _c.c_int), ("depth", _c.c_int),
+                ("kernel", _c.c_int)]
 
 
 class EvolveStats(_c.Structure):

@@ -33,6 +33,11 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-result",
     "-I" + os.path.join(ROOT, "include"),
 ]
+# per-source additions: the apply kernels branch on uniform values (the row
+# flip of a sub, its class run) and are compiled without the structurizer's
+# rewrite of uniform regions, which otherwise routes every accumulator of a
+# multi-way uniform branch through copies at its join
+HIP_FILE_FLAGS = {"rp_apply.hip": ["-mllvm", "-structurizecfg-skip-uniform-regions"]}
 CXX = os.environ.get("CXX", "g++")
 HOST_FLAGS = [
     "-O2", "-fPIC", "-std=c++20", "-fopenmp", "-ffp-contract=off", "-Wall",
@@ -71,7 +76,7 @@ def build_hip(force=False):
         s = os.path.join(CSRC, src)
         o = os.path.join(BUILD, src.replace(".hip", ".o"))
         if force or _stale(o, [s] + deps):
-            _run([HIPCC] + HIP_FLAGS + ["-c", s, "-o", o])
+            _run([HIPCC] + HIP_FLAGS + HIP_FILE_FLAGS.get(src, []) + ["-c", s, "-o", o])
         objs.append(o)
     lib = os.path.join(HERE, "libreapers_hip.so")
     if force or _stale(lib, objs):

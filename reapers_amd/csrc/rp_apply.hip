@@ -57,6 +57,7 @@ struct rp_terms {
     int group_acc = 0;       // FP32 within a group, FP64 across groups (complex<float>)
     uint64_t nterms = 0, nflips = 0, ngroups = 0, nsubs = 0;
     void *d_groups = nullptr, *d_subs = nullptr, *d_terms = nullptr, *d_flat = nullptr;
+    void *d_cgroups = nullptr, *d_csubs = nullptr, *d_cterms = nullptr;   // compact stream
     void *d_split = nullptr;          // per-slice partial sums of the split small kernel
     size_t split_bytes = 0;
     std::vector<uint64_t> raw_bits;   // the uploaded term list, for re-tiling
@@ -536,6 +537,265 @@ k_apply_tiled(const typename CxA<R>::T *__restrict__ in, typename CxA<R>::T *__r
 }
 
 // ---------------------------------------------------------------------------
+// The complex<float> group-accumulating apply over compact sub records.
+// The 48-byte SubRec list of a sparse SYK Hamiltonian at 24-30 qubits
+// (370-480 subs, 18-23 KiB) does not fit the 16 KiB scalar data cache, so
+// every sub record of the kernel above is a dependent load from L2.  Here
+// each sub is 16 bytes (6-8 KiB); the subs of all groups form one list in
+// group order (the GroupRuns bounds index it as they index SubRec), and
+// the record of the next sub, and the header of the group after next, are
+// loaded while the current ones are in use.
+// ---------------------------------------------------------------------------
+struct SubC { uint32_t xl, z; float c0, c1; };
+struct TermC { uint32_t z; float cr, ci; uint32_t pad; };
+// group header: tile index above the tile bits, first sub, subs per class
+// run (kRunReal0 .. kRunMulti)
+struct GroupC { uint32_t xh, begin; uint8_t n[kRuns]; uint8_t pad[8 - kRuns]; };
+static_assert(sizeof(GroupC) == 16, "16-byte group header");
+
+template <int V> struct IC { static constexpr int value = V; };
+
+// f(IC<xr>) for the run-time row flip xr < RP: one code path per value, so
+// that the row offsets are immediates
+template <int RP, typename F>
+__device__ __forceinline__ void with_row_flip(uint32_t xr, F &&f)
+{
+    if constexpr (RP == 8) {
+        switch (xr) {
+        case 0: f(IC<0>{}); break;
+        case 1: f(IC<1>{}); break;
+        case 2: f(IC<2>{}); break;
+        case 3: f(IC<3>{}); break;
+        case 4: f(IC<4>{}); break;
+        case 5: f(IC<5>{}); break;
+        case 6: f(IC<6>{}); break;
+        default: f(IC<7>{}); break;
+        }
+    } else if constexpr (RP == 4) {
+        switch (xr) {
+        case 0: f(IC<0>{}); break;
+        case 1: f(IC<1>{}); break;
+        case 2: f(IC<2>{}); break;
+        default: f(IC<3>{}); break;
+        }
+    } else if constexpr (RP == 2) {
+        if (xr) f(IC<1>{});
+        else f(IC<0>{});
+    } else {
+        f(IC<0>{});
+    }
+}
+
+// all RP row values of a sub loaded before any is used: the reads go out
+// back to back and one wait covers them
+template <int RP> __device__ __forceinline__ void pin_rows(float2 (&v)[RP])
+{
+    if constexpr (RP == 8)
+        asm volatile("" : "+v"(v[0].x), "+v"(v[0].y), "+v"(v[1].x), "+v"(v[1].y), "+v"(v[2].x),
+                     "+v"(v[2].y), "+v"(v[3].x), "+v"(v[3].y), "+v"(v[4].x), "+v"(v[4].y),
+                     "+v"(v[5].x), "+v"(v[5].y), "+v"(v[6].x), "+v"(v[6].y), "+v"(v[7].x),
+                     "+v"(v[7].y));
+    else if constexpr (RP == 4)
+        asm volatile("" : "+v"(v[0].x), "+v"(v[0].y), "+v"(v[1].x), "+v"(v[1].y), "+v"(v[2].x),
+                     "+v"(v[2].y), "+v"(v[3].x), "+v"(v[3].y));
+    else if constexpr (RP == 2)
+        asm volatile("" : "+v"(v[0].x), "+v"(v[0].y), "+v"(v[1].x), "+v"(v[1].y));
+    else
+        asm volatile("" : "+v"(v[0].x), "+v"(v[0].y));
+}
+
+// One single-term sub with the row flip XR fixed: input row j of the
+// lane's column (row base `row`, immediate offsets) feeds output row
+// j ^ XR.  c (and ci for the complex class) carry the lane's phase sign; zr
+// = the phase bits among the row bits (classes 1 and complex).
+template <int RP, int NT, uint32_t CLS, int XR>
+__device__ __forceinline__ void sub_rows(const float2 *row, float c, float ci, uint32_t zr,
+                                         float (&gr)[RP], float (&gi)[RP])
+{
+    float2 vv[RP];
+#pragma unroll
+    for (int j = 0; j < RP; j++) vv[j] = row[j * NT];
+    pin_rows<RP>(vv);
+#pragma unroll
+    for (int j = 0; j < RP; j++) {
+        const int r = j ^ XR;
+        const float2 v = vv[j];
+        // explicit fused multiply-adds: one rounding each, and no product
+        // that the compiler could share between the row-flip paths
+        if constexpr (CLS == kRunReal0) {
+            gr[r] = __builtin_fmaf(c, v.x, gr[r]);
+            gi[r] = __builtin_fmaf(c, v.y, gi[r]);
+        } else if constexpr (CLS == kRunImag0) {
+            gr[r] = __builtin_fmaf(-c, v.y, gr[r]);
+            gi[r] = __builtin_fmaf(c, v.x, gi[r]);
+        } else if constexpr (CLS == kRunReal1) {
+            const float cc = flip_sign(c, parity_sign((uint32_t)r & zr));
+            gr[r] = __builtin_fmaf(cc, v.x, gr[r]);
+            gi[r] = __builtin_fmaf(cc, v.y, gi[r]);
+        } else if constexpr (CLS == kRunImag1) {
+            const float cc = flip_sign(c, parity_sign((uint32_t)r & zr));
+            gr[r] = __builtin_fmaf(-cc, v.y, gr[r]);
+            gi[r] = __builtin_fmaf(cc, v.x, gi[r]);
+        } else {
+            const uint32_t rsb = parity_sign((uint32_t)r & zr);
+            const float a = flip_sign(c, rsb), b = flip_sign(ci, rsb);
+            gr[r] += a * v.x - b * v.y;
+            gi[r] += a * v.y + b * v.x;
+        }
+    }
+}
+
+// The subs of one class run [s, end) of a group from the staged tile buf.
+// c0 holds the real coefficient (real classes), the imaginary one
+// (imaginary classes) or the real part with c1 the imaginary part
+// (complex); a multi sub holds its term range [z, bits of c0) in the FP32
+// term list.  S is the record at s on entry and at end on exit.
+template <int TB, int NT, uint32_t CLS>
+__device__ __forceinline__ void stream_run(uint32_t &s, uint32_t end, SubC &S,
+                                           const SubC *__restrict__ subs, const float2 *buf,
+                                           uint32_t lane8, uint32_t lanebits,
+                                           const TermC *__restrict__ terms,
+                                           float (&gr)[(1 << TB) / NT], float (&gi)[(1 << TB) / NT])
+{
+    using T = float2;
+    constexpr int RP = (1 << TB) / NT;
+    constexpr int LB = NT == 1024 ? 10 : NT == 512 ? 9 : NT == 256 ? 8 : 7;
+    for (; s < end; s++) {
+        const SubC Sn = subs[s + 1];     // the list has one sentinel record
+        const T *row = sub_lane_base<T, NT>(buf, lane8, S.xl);
+        const uint32_t xr = S.xl >> LB;
+        if constexpr (CLS != kRunMulti) {
+            const uint32_t lsb = parity_sign(lanebits & S.z);
+            const uint32_t zr = (S.z >> LB) & (RP - 1);
+            float c = flip_sign(S.c0, lsb), ci = 0.f;
+            // a lane-signed coefficient that each row flips again stays in a
+            // register (one xor per row instead of the popcount per row)
+            if constexpr (CLS == kRunReal1 || CLS == kRunImag1 || CLS == kRunCplx) c = opaque(c);
+            if constexpr (CLS == kRunCplx) ci = opaque(flip_sign(S.c1, lsb));
+            with_row_flip<RP>(xr, [&](auto X) {
+                sub_rows<RP, NT, CLS, decltype(X)::value>(row, c, ci, zr, gr, gi);
+            });
+        } else {
+            constexpr int RC = RP < 4 ? RP : 4;
+            const uint32_t tb = S.z, te = __float_as_uint(S.c0);
+#pragma unroll
+            for (int r0 = 0; r0 < RP; r0 += RC) {
+                float dr[RC], di[RC];
+#pragma unroll
+                for (int q = 0; q < RC; q++) { dr[q] = 0; di[q] = 0; }
+                for (uint32_t t = tb; t < te; t++) {
+                    const TermC C = terms[t];
+                    const uint32_t lsb = parity_sign(lanebits & C.z);
+                    const float cr = flip_sign(C.cr, lsb), ci = flip_sign(C.ci, lsb);
+                    const uint32_t zr = (C.z >> LB) & (RP - 1);
+#pragma unroll
+                    for (int q = 0; q < RC; q++) {
+                        const uint32_t rsb = parity_sign((r0 + q) & zr);
+                        dr[q] += flip_sign(cr, rsb);
+                        di[q] += flip_sign(ci, rsb);
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < RC; q++) {
+                    const T v = row[((r0 + q) ^ xr) * NT];
+                    gr[r0 + q] += dr[q] * v.x - di[q] * v.y;
+                    gi[r0 + q] += dr[q] * v.y + di[q] * v.x;
+                }
+            }
+        }
+        S = Sn;
+    }
+}
+
+// one group of the streamed kernel: wait for its tile (buffer `cur`),
+// start the next group's tile into `nxt`, apply the group's subs run by
+// run.  Gc / Gn: this and the next group's headers (the header list has two
+// sentinel entries); S the record of the group's first sub on entry and of
+// the next group's first sub on exit.
+template <int TB, int NT>
+__device__ __forceinline__ void stream_phase(int g, int ngroups, const float2 *__restrict__ in,
+                                             const GroupC *__restrict__ groups,
+                                             const SubC *__restrict__ subs,
+                                             const TermC *__restrict__ terms, uint32_t t0,
+                                             uint32_t tid, const float2 *cur, float2 *nxt,
+                                             SubC &S, uint32_t &s, GroupC &Gc, GroupC &Gn,
+                                             float (&gr)[(1 << TB) / NT],
+                                             float (&gi)[(1 << TB) / NT])
+{
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (g + 1 < ngroups) tile_dma<float2, TB, NT>(in, t0 ^ (Gn.xh << TB), nxt);
+    const GroupC Gnn = groups[g + 2];
+    const uint32_t lane8 = tid * (uint32_t)sizeof(float2), lb = t0 | tid;
+    uint32_t e = Gc.begin;
+#define RP_STREAM_RUN(CLS)                                                                  \
+    e += Gc.n[CLS];                                                                         \
+    stream_run<TB, NT, CLS>(s, e, S, subs, cur, lane8, lb, terms, gr, gi);
+    RP_STREAM_RUN(kRunReal0)
+    RP_STREAM_RUN(kRunReal1)
+    RP_STREAM_RUN(kRunImag0)
+    RP_STREAM_RUN(kRunImag1)
+    RP_STREAM_RUN(kRunCplx)
+    RP_STREAM_RUN(kRunMulti)
+#undef RP_STREAM_RUN
+    Gc = Gn;
+    Gn = Gnn;
+}
+
+template <int TB, int NT>
+__global__ void __launch_bounds__(NT, (apply_waves_per_eu<float, TB, NT, 2>()))
+k_apply_stream(const float2 *__restrict__ in, float2 *__restrict__ out,
+               const GroupC *__restrict__ groups, int ngroups, const SubC *__restrict__ subs,
+               const TermC *__restrict__ terms, int accumulate)
+{
+    using T = float2;
+    constexpr int TILE = 1 << TB;
+    constexpr int RP = TILE / NT;
+    __shared__ T tile0[TILE];
+    __shared__ T tile1[TILE];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t t0 = xcd_tile_index() << TB;
+    double accr[RP], acci[RP];
+    float gr[RP], gi[RP];
+#pragma unroll
+    for (int r = 0; r < RP; r++) {
+        accr[r] = 0;
+        acci[r] = 0;
+        gr[r] = 0;
+        gi[r] = 0;
+        asm volatile("" : "+v"(gr[r]), "+v"(gi[r]));
+    }
+    GroupC Gc = groups[0], Gn = groups[1];
+    uint32_t s = 0;
+    SubC S = subs[0];
+    tile_dma<T, TB, NT>(in, t0 ^ (Gc.xh << TB), tile0);
+    for (int g = 0; g < ngroups; g += 2) {
+        stream_phase<TB, NT>(g, ngroups, in, groups, subs, terms, t0, tid, tile0, tile1, S, s, Gc,
+                             Gn, gr, gi);
+        if (g + 1 >= ngroups) break;
+        stream_phase<TB, NT>(g + 1, ngroups, in, groups, subs, terms, t0, tid, tile1, tile0, S, s,
+                             Gc, Gn, gr, gi);
+        // FP32 partial sums of every four groups into the FP64 accumulators
+        if ((g / 2) & 1) flush_partials(accr, acci, gr, gi);
+    }
+    flush_partials(accr, acci, gr, gi);
+#pragma unroll
+    for (int r = 0; r < RP; r++) {
+        const uint64_t m = (uint64_t)t0 + r * NT + tid;
+        T o;
+        if (accumulate) {
+            const T p = out[m];
+            o.x = (float)(accr[r] + (double)p.x);
+            o.y = (float)(acci[r] + (double)p.y);
+        } else {
+            o.x = (float)accr[r];
+            o.y = (float)acci[r];
+        }
+        out[m] = o;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // The apply with the first Gram-Schmidt pass's coefficients formed in its
 // epilogue: after writing its output tile w, the workgroup reads the same
 // rows of V_0..V_{k-1} and writes its FP64 partials of <V_q|w> (columns
@@ -651,9 +911,9 @@ static bool tiling_instantiated(int prec, int tb, int nt, int d)
 // count up to 128 that keeps at most 2048 workgroups and 32 terms per slice.
 // Beyond, the tiled kernel: 8 amplitudes per thread, smaller tiles and
 // workgroups for shorter chains so that there are enough workgroups.
-static void choose_tiling(int prec, int len, uint64_t nterms, int *tb, int *nt, int *depth)
+static void choose_tiling(int prec, int len, uint64_t nterms, int group_acc, int *tb, int *nt,
+                          int *depth)
 {
-    (void)prec;
     *depth = 2;
     if (len <= kSplitMaxLen) {
         uint64_t blocks = std::max<uint64_t>(1, (1ull << len) / kSmallThreads);
@@ -669,11 +929,59 @@ static void choose_tiling(int prec, int len, uint64_t nterms, int *tb, int *nt, 
     }
     // 2^12-amplitude tiles with 512 threads: for complex<float> sparse SYK at
     // 24 qubits faster than 2^13 tiles (more groups, but two workgroups per
-    // CU instead of one); for complex<double> the largest tile in 128 KiB
+    // CU instead of one); for complex<double> the largest tile in 128 KiB.
+    // For complex<float> with group accumulation from 28 sites (states of
+    // 2 GiB and more, beyond every cache) 2^13 tiles with 1024 threads:
+    // sparse SYK at 30 qubits takes 197 ms there against 260 ms with 2^12
+    // tiles
+    if (prec == RP_C64 && group_acc && len >= 28) { *tb = 13; *nt = 1024; return; }
     if (len >= 18) { *tb = 12; *nt = 512; return; }
     if (len >= 14) { *tb = 10; *nt = 128; return; }
     *tb = 8;
     *nt = 128;
+}
+
+// the compact lists of the tiled tables (complex<float> states, FP64 term
+// coefficients): one 16-byte record per sub in the SubRec order,
+// coefficients rounded to float, one sentinel record; one 16-byte header
+// per group, two sentinel headers; the multi-sub terms in FP32.  False when
+// a class run of some group has more than 255 subs (the header's count
+// width): the handle then keeps the SubRec kernel.
+static bool compact_stream(const Tables<double> &tab, std::vector<GroupC> &cg,
+                           std::vector<SubC> &cs, std::vector<TermC> &ct)
+{
+    for (const TermRec<double> &t : tab.terms) ct.push_back({t.z, (float)t.cr, (float)t.ci, 0});
+    for (const GroupRuns &G : tab.groups) {
+        GroupC h{};
+        h.xh = G.xh;
+        h.begin = G.b[0];
+        for (uint32_t k = 0; k < kRuns; k++) {
+            if (G.b[k + 1] - G.b[k] > 255) return false;
+            h.n[k] = (uint8_t)(G.b[k + 1] - G.b[k]);
+            for (uint32_t i = G.b[k]; i < G.b[k + 1]; i++) {
+                const SubRec<double> &S = tab.subs[i];
+                SubC c{S.xl, S.z, 0.f, 0.f};
+                if (k == kRunMulti) {
+                    c.z = S.term_begin;
+                    memcpy(&c.c0, &S.term_end, sizeof(float));
+                } else if (k == kRunImag0 || k == kRunImag1) {
+                    c.c0 = S.fi;
+                } else {
+                    c.c0 = S.fr;
+                    c.c1 = S.fi;
+                }
+                cs.push_back(c);
+            }
+        }
+        cg.push_back(h);
+    }
+    GroupC end{};
+    end.begin = (uint32_t)cs.size();
+    cg.push_back(end);
+    cg.push_back(end);
+    cs.push_back({0, 0, 0.f, 0.f});
+    if (ct.empty()) ct.push_back({0, 0.f, 0.f, 0});
+    return true;
 }
 
 template <typename R, typename A>
@@ -725,6 +1033,16 @@ static int upload_impl(rp_terms *h, const uint64_t *bits, const double *coeff, u
         if ((st = up(&h->d_subs, subs.data(), subs.size() * sizeof(SubRec<A>)))) return st;
         if ((st = up(&h->d_terms, terms.data(), terms.size() * sizeof(TermRec<A>)))) return st;
     }
+    if constexpr (sizeof(R) == 4 && sizeof(A) == 8) {
+        std::vector<GroupC> cg;
+        std::vector<SubC> cs;
+        std::vector<TermC> ct;
+        if (tb && compact_stream(tab, cg, cs, ct)) {
+            if ((st = up(&h->d_cgroups, cg.data(), cg.size() * sizeof(GroupC)))) return st;
+            if ((st = up(&h->d_csubs, cs.data(), cs.size() * sizeof(SubC)))) return st;
+            if ((st = up(&h->d_cterms, ct.data(), ct.size() * sizeof(TermC)))) return st;
+        }
+    }
     RP_CHECK_HIP(hipStreamSynchronize(c.stream));
     return RP_OK;
 }
@@ -737,6 +1055,8 @@ static void free_handle(rp_terms *h)
     if (h->d_terms) (void)hipFree(h->d_terms);
     if (h->d_flat) (void)hipFree(h->d_flat);
     if (h->d_split) (void)hipFree(h->d_split);
+    for (void *p : {h->d_cgroups, h->d_csubs, h->d_cterms})
+        if (p) (void)hipFree(p);
     delete h;
 }
 
@@ -746,6 +1066,12 @@ static int launch_tiled(rp_terms *h, void *res, const void *vec, int accumulate)
     using T = typename CxA<R>::T;
     const uint64_t grid = (1ull << h->len) >> TB;
     if constexpr (sizeof(R) == 4 && sizeof(A) == 8 && D == 2) {
+        if (h->group_acc && h->d_csubs) {
+            RP_LAUNCH(RP_KERNEL_APPLY, (k_apply_stream<TB, NT>), dim3((unsigned)grid), dim3(NT), 0,
+                      (const T *)vec, (T *)res, (const GroupC *)h->d_cgroups, (int)h->ngroups,
+                      (const SubC *)h->d_csubs, (const TermC *)h->d_cterms, accumulate);
+            return RP_OK;
+        }
         if (h->group_acc) {
             RP_LAUNCH(RP_KERNEL_APPLY, (k_apply_tiled<R, A, TB, NT, D, true>), dim3((unsigned)grid),
                       dim3(NT), 0, (const T *)vec, (T *)res, (const GroupRuns *)h->d_groups,
@@ -858,7 +1184,8 @@ static int upload_any(rp_terms *h, const uint64_t *bits, const double *coeff, ui
 
 static void free_tables(rp_terms *h)
 {
-    for (void **p : {&h->d_groups, &h->d_subs, &h->d_terms, &h->d_flat})
+    for (void **p : {&h->d_groups, &h->d_subs, &h->d_terms, &h->d_flat, &h->d_cgroups, &h->d_csubs,
+                     &h->d_cterms})
         if (*p) { (void)hipFree(*p); *p = nullptr; }
 }
 
@@ -946,7 +1273,7 @@ int rp_terms_upload_ex(rp_terms_t *out, int prec, int len, const uint64_t *bits,
     h->len = len;
     h->acc64 = (prec == RP_C128 || !(flags & RP_TERMS_ACC_NATIVE)) ? 1 : 0;
     h->group_acc = (prec == RP_C64 && (flags & RP_TERMS_ACC_GROUP)) ? 1 : 0;
-    choose_tiling(prec, len, nterms, &h->tile_bits, &h->threads, &h->depth);
+    choose_tiling(prec, len, nterms, h->group_acc, &h->tile_bits, &h->threads, &h->depth);
     h->raw_bits.assign(bits, bits + nterms);
     h->raw_coeff.assign(coeff, coeff + 2 * nterms);
     st = upload_any(h, bits, coeff, nterms);
@@ -1003,6 +1330,10 @@ int rp_terms_info(rp_terms_t h, rp_terms_info_t *info)
     info->nterms = h->nterms;
     info->nflips = h->nflips;
     info->ngroups = h->tile_bits ? h->ngroups : h->nflips;
+    info->kernel = h->tile_bits == 0 ? RP_APPLY_SMALL
+                   : (h->prec == RP_C64 && h->acc64 && h->group_acc && h->depth == 2 && h->d_csubs)
+                       ? RP_APPLY_STREAM
+                       : RP_APPLY_TILED;
     info->bytes_per_apply = (info->ngroups + 1) * (1ull << h->len) * elem_bytes(h->prec);
     return RP_OK;
 }

@@ -204,3 +204,60 @@ def test_default_split_tiling_dense_syk24(gpu, orc, fp):
     err = np.abs(s.store().astype(np.complex128) - ref).max() / np.abs(ref).max()
     print("dense SYK N=24 fp%d, %d slices: %.2e" % (fp, info.depth, err), flush=True)
     assert err < (1e-6 if fp == 32 else 1e-13)
+
+
+@pytest.mark.parametrize("tiling", [(12, 512, 2), (12, 1024, 2), (13, 1024, 2), (10, 128, 2),
+                                    (8, 128, 2)])
+def test_group_accumulation_every_sub_class(gpu, orc, tiling):
+    """RP_TERMS_ACC_GROUP on a term list with every kind of sub the tiled
+    tables store: single terms with real, imaginary and complex
+    coefficients, with and without phase bits among the row bits, and flip
+    masks shared by several terms; a plain and an accumulating apply
+    against the FP64 oracle."""
+    from gpu_util import DeviceBuffer
+    import ctypes
+    L = 18
+    rng = np.random.default_rng(77 + tiling[0] + tiling[1] + tiling[2])
+    bits, coeff = random_pauli_sum(rng, L, 240, max_sites=4)
+    coeff[0::3] = coeff[0::3].real
+    coeff[1::3] = 1j * coeff[1::3].imag
+    # 60 terms that share a flip mask with an earlier one: the Z bit of one
+    # of its sites toggled (X <-> Y, I <-> Z) keeps x and changes z
+    extra = []
+    for i in rng.choice(len(bits), 60, replace=False):
+        s = int(rng.integers(0, L))
+        extra.append(int(bits[i]) ^ (2 << (2 * s)))
+    bits = np.concatenate([bits, np.array(extra, np.uint64)])
+    coeff = np.concatenate([coeff, rng.standard_normal(60) + 1j * rng.standard_normal(60)])
+    coeff = coeff.astype(np.complex64).astype(np.complex128)
+    psi = random_state(rng, L, np.complex64)
+    prior = random_state(rng, L, np.complex64)
+    ref = orc.apply(L, bits, coeff, psi.astype(np.complex128), fp=64)
+    b = np.ascontiguousarray(bits, np.uint64)
+    co = np.empty(2 * len(b))
+    co[0::2], co[1::2] = np.real(coeff), np.imag(coeff)
+    h = ctypes.c_void_p()
+    hip = gpu.hip()
+    gpu.check(hip.rp_terms_upload_ex(ctypes.byref(h), gpu.RP_C64, L,
+                                     b.ctypes.data_as(ctypes.c_void_p),
+                                     co.ctypes.data_as(ctypes.c_void_p), len(b),
+                                     gpu.RP_TERMS_ACC_GROUP))
+    gpu.check(hip.rp_terms_set_tiling(h, *tiling))
+    vin = DeviceBuffer.from_array(psi)
+    vout = DeviceBuffer(psi.nbytes)
+    gpu.check(hip.rp_apply_ops(h, vout.ptr, vin.ptr, 0))
+    got = vout.to_array(np.complex64, len(psi)).astype(np.complex128)
+    scale = np.abs(ref).max()
+    err = np.abs(got - ref).max() / scale
+    vacc = DeviceBuffer.from_array(prior)
+    gpu.check(hip.rp_apply_ops(h, vacc.ptr, vin.ptr, 1))
+    got_acc = vacc.to_array(np.complex64, len(psi)).astype(np.complex128)
+    err_acc = np.abs(got_acc - (ref + prior.astype(np.complex128))).max() / scale
+    print("every sub class, tiling %s: %d terms, rel err %.2e, accumulating %.2e"
+          % (tiling, len(b), err, err_acc), flush=True)
+    for buf in (vin, vout, vacc):
+        buf.free()
+    gpu.check(hip.rp_terms_free(h))
+    # FP32 sums of up to a few dozen terms per group, FP64 across groups
+    assert err < 2e-6
+    assert err_acc < 2e-6
