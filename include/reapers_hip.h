@@ -151,6 +151,9 @@ typedef struct {
     int      depth;         /* LDS tiles in its pipeline (input tiles in flight + 1);
                                for tile_bits 0 the number of term slices */
     int      kernel;        /* RP_APPLY_SMALL, RP_APPLY_TILED or RP_APPLY_STREAM */
+    double   apply_passes;  /* the kept tiling's apply time over one streaming pass
+                               of the state (half an rp_scale_vec), measured by the
+                               autotuner at upload; 0 without RP_TERMS_AUTOTUNE */
 } rp_terms_info_t;
 /* the apply kernel family a term handle launches: one thread per amplitude
    (tile_bits 0); tiles with per-sub records; tiles with the compact sub

@@ -60,6 +60,7 @@ struct rp_terms {
     void *d_cgroups = nullptr, *d_csubs = nullptr, *d_cterms = nullptr;   // compact stream
     void *d_split = nullptr;          // per-slice partial sums of the split small kernel
     size_t split_bytes = 0;
+    double apply_passes = 0;          // measured apply cost in state passes (autotune)
     std::vector<uint64_t> raw_bits;   // the uploaded term list, for re-tiling
     std::vector<double> raw_coeff;
 };
@@ -1241,6 +1242,20 @@ static int autotune(rp_terms *h)
         }
         if (ms < best) { best = ms; pick = cd; }
     }
+    // the kept tiling's apply against one streaming pass of the same vector
+    // (3 scalings of x, each reading and writing it: 6 passes)
+    // (3 scalings of x, each reading and writing it: 6 passes); a failure
+    // here leaves the figure at 0 and the upload as it is
+    if (!st && best < 1e30f) {
+        float ms = 0;
+        int sst = hipEventRecord(e0, c.stream) == hipSuccess ? RP_OK : RP_ERR_RUNTIME;
+        for (int r = 0; r < 3 && !sst; r++)
+            sst = rp_scale_vec(h->prec, x, 1.0, 0.0, 1ull << h->len);
+        if (!sst && hipEventRecord(e1, c.stream) == hipSuccess &&
+            hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess &&
+            ms > 0)
+            h->apply_passes = std::max(1.0, (double)best / 3.0 / ((double)ms / 6.0));
+    }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     (void)hipStreamSynchronize(c.stream);
@@ -1330,6 +1345,7 @@ int rp_terms_info(rp_terms_t h, rp_terms_info_t *info)
     info->nterms = h->nterms;
     info->nflips = h->nflips;
     info->ngroups = h->tile_bits ? h->ngroups : h->nflips;
+    info->apply_passes = h->apply_passes;
     info->kernel = h->tile_bits == 0 ? RP_APPLY_SMALL
                    : (h->prec == RP_C64 && h->acc64 && h->group_acc && h->depth == 2 && h->d_csubs)
                        ? RP_APPLY_STREAM

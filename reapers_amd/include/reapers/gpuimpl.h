@@ -386,9 +386,13 @@ public:
     // HBM traffic of one apply in units of one state vector (the uploaded
     // layout's tile groups plus the output write)
     template<RealScalar FpType>
+    // the apply's cost in state-vector passes: as the autotuner measured it
+    // at upload (kept tiling's time over one streaming pass), else the tile
+    // stream's count G_b + 1
     static double apply_passes(typename SpinOp<FpType>::IndexType len,
                                const DevSumOps<FpType> &ops) {
         rp_terms_info_t info = ops.device_info(len);
+        if (info.apply_passes > 0) return info.apply_passes;
         double vec = (double)((size_t)1 << len) * 2 * sizeof(FpType);
         return std::max(1.0, (double)info.bytes_per_apply / vec);
     }
