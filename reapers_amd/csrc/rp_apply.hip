@@ -542,10 +542,12 @@ k_apply_tiled(const typename CxA<R>::T *__restrict__ in, typename CxA<R>::T *__r
 // The 48-byte SubRec list of a sparse SYK Hamiltonian at 24-30 qubits
 // (370-480 subs, 18-23 KiB) does not fit the 16 KiB scalar data cache, so
 // every sub record of the kernel above is a dependent load from L2.  Here
-// each sub is 16 bytes (6-8 KiB); the subs of all groups form one list in
-// group order (the GroupRuns bounds index it as they index SubRec), and
-// the record of the next sub, and the header of the group after next, are
-// loaded while the current ones are in use.
+// each sub is 16 bytes and each group header 16 bytes (8-12 KiB together);
+// the subs of all groups form one list in group order, and the record of
+// the next sub, and the header of the group after next, are loaded while
+// the current ones are in use.  Each single-term sub branches once on its
+// row flip (uniform) to a code path whose LDS row reads are immediate
+// offsets from one lane address.
 // ---------------------------------------------------------------------------
 struct SubC { uint32_t xl, z; float c0, c1; };
 struct TermC { uint32_t z; float cr, ci; uint32_t pad; };
@@ -743,19 +745,18 @@ __device__ __forceinline__ void stream_phase(int g, int ngroups, const float2 *_
     Gn = Gnn;
 }
 
+// the whole output tile t0 of the streamed kernel: every group's input tile
+// through the two LDS tiles, then the tile written once
 template <int TB, int NT>
-__global__ void __launch_bounds__(NT, (apply_waves_per_eu<float, TB, NT, 2>()))
-k_apply_stream(const float2 *__restrict__ in, float2 *__restrict__ out,
-               const GroupC *__restrict__ groups, int ngroups, const SubC *__restrict__ subs,
-               const TermC *__restrict__ terms, int accumulate)
+__device__ __forceinline__ void stream_tile(const float2 *__restrict__ in, float2 *__restrict__ out,
+                                            const GroupC *__restrict__ groups, int ngroups,
+                                            const SubC *__restrict__ subs,
+                                            const TermC *__restrict__ terms, int accumulate,
+                                            uint32_t t0, float2 *tile0, float2 *tile1)
 {
     using T = float2;
-    constexpr int TILE = 1 << TB;
-    constexpr int RP = TILE / NT;
-    __shared__ T tile0[TILE];
-    __shared__ T tile1[TILE];
+    constexpr int RP = (1 << TB) / NT;
     const uint32_t tid = threadIdx.x;
-    const uint32_t t0 = xcd_tile_index() << TB;
     double accr[RP], acci[RP];
     float gr[RP], gi[RP];
 #pragma unroll
@@ -794,6 +795,18 @@ k_apply_stream(const float2 *__restrict__ in, float2 *__restrict__ out,
         }
         out[m] = o;
     }
+}
+
+template <int TB, int NT>
+__global__ void __launch_bounds__(NT, (apply_waves_per_eu<float, TB, NT, 2>()))
+k_apply_stream(const float2 *__restrict__ in, float2 *__restrict__ out,
+               const GroupC *__restrict__ groups, int ngroups, const SubC *__restrict__ subs,
+               const TermC *__restrict__ terms, int accumulate)
+{
+    __shared__ float2 tile0[1 << TB];
+    __shared__ float2 tile1[1 << TB];
+    stream_tile<TB, NT>(in, out, groups, ngroups, subs, terms, accumulate, xcd_tile_index() << TB,
+                        tile0, tile1);
 }
 
 // ---------------------------------------------------------------------------
