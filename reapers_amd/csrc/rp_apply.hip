@@ -220,6 +220,10 @@ __device__ __forceinline__ double flip_sign(double c, uint32_t sb)
 {
     return __longlong_as_double(__double_as_longlong(c) ^ ((long long)sb << 32));
 }
+// a * b + c with one rounding
+__device__ __forceinline__ float fmadd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmadd(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 // the lane-signed coefficient of a sub whose rows flip it again: kept in a
 // register, so that each row's flip is one xor with a scalar sign instead
 // of the whole popcount per row
@@ -317,8 +321,10 @@ __device__ __forceinline__ void apply_group(const GroupRuns &G, const typename C
             const T v = row[(r ^ xr) * NT];
             const uint32_t rsb = parity_sign(r & zr);
             const P a = flip_sign(cr, rsb), c = flip_sign(ci, rsb);
-            accr[r] += a * (P)v.x - c * (P)v.y;
-            acci[r] += a * (P)v.y + c * (P)v.x;
+            // two nested fused multiply-adds per component (the rounding
+            // order of the compact-record kernel)
+            accr[r] = fmadd(a, (P)v.x, fmadd(-c, (P)v.y, accr[r]));
+            acci[r] = fmadd(a, (P)v.y, fmadd(c, (P)v.x, acci[r]));
         }
     }
 #undef RP_SUB_PROLOGUE
@@ -350,8 +356,8 @@ __device__ __forceinline__ void apply_group(const GroupRuns &G, const typename C
 #pragma unroll
             for (int q = 0; q < RC; q++) {
                 const T v = row[((r0 + q) ^ xr) * NT];
-                accr[r0 + q] += dr[q] * (P)v.x - di[q] * (P)v.y;
-                acci[r0 + q] += dr[q] * (P)v.y + di[q] * (P)v.x;
+                accr[r0 + q] = fmadd(dr[q], (P)v.x, fmadd(-di[q], (P)v.y, accr[r0 + q]));
+                acci[r0 + q] = fmadd(dr[q], (P)v.y, fmadd(di[q], (P)v.x, acci[r0 + q]));
             }
         }
     }
@@ -642,8 +648,8 @@ __device__ __forceinline__ void sub_rows(const float2 *row, float c, float ci, u
         } else {
             const uint32_t rsb = parity_sign((uint32_t)r & zr);
             const float a = flip_sign(c, rsb), b = flip_sign(ci, rsb);
-            gr[r] += a * v.x - b * v.y;
-            gi[r] += a * v.y + b * v.x;
+            gr[r] = __builtin_fmaf(a, v.x, __builtin_fmaf(-b, v.y, gr[r]));
+            gi[r] = __builtin_fmaf(a, v.y, __builtin_fmaf(b, v.x, gi[r]));
         }
     }
 }
@@ -701,8 +707,8 @@ __device__ __forceinline__ void stream_run(uint32_t &s, uint32_t end, SubC &S,
 #pragma unroll
                 for (int q = 0; q < RC; q++) {
                     const T v = row[((r0 + q) ^ xr) * NT];
-                    gr[r0 + q] += dr[q] * v.x - di[q] * v.y;
-                    gi[r0 + q] += dr[q] * v.y + di[q] * v.x;
+                    gr[r0 + q] = __builtin_fmaf(dr[q], v.x, __builtin_fmaf(-di[q], v.y, gr[r0 + q]));
+                    gi[r0 + q] = __builtin_fmaf(dr[q], v.y, __builtin_fmaf(di[q], v.x, gi[r0 + q]));
                 }
             }
         }
