@@ -226,7 +226,8 @@ int rp_gs_pass_fused(int prec, const void *const *V, int k, void *w, uint64_t n,
  * overlap on one GPU (from several host threads or streams) race. */
 int rp_gs_pass_fused_noalloc(int prec, const void *const *V, int k, void *w, uint64_t n,
                              double *h, double *norm, double fuse_min, int *normalized);
-/* y += sum_i c[i] V[i] for k vectors, c given as 2k doubles (re, im) */
+/* y += sum_i c[i] V[i] for k vectors (any k; one sweep per 16 of them), c
+   given as 2k doubles (re, im) */
 int rp_lincomb(int prec, void *y, const void *const *V, int k, const double *c, uint64_t n);
 
 /* ---------------------------------------------------------------------- */

@@ -834,7 +834,7 @@ int rp_lincomb(int prec, void *y, const void *const *V, int k, const double *c, 
     int st = check_init();
     if (st) return st;
     RP_REQUIRE(valid_prec(prec), "bad precision");
-    RP_REQUIRE(k >= 0 && k <= kMaxVecs, "k must be in [0, 64]");
+    RP_REQUIRE(k >= 0, "k must not be negative");
     if (n == 0 || k == 0) return RP_OK;
     RP_REQUIRE(y && V && c, "null argument");
     for (int q = 0; q < k; q++) RP_REQUIRE(V[q], "null basis vector");

@@ -100,3 +100,21 @@ def test_xxz_evolution(gpu, orc):
     s.load(psi)
     s.evolve(ham, t, 0.2, krydim=12, tol=1e-13)
     assert np.abs(s.store() - ex).max() < 1e-11
+
+
+def test_lanczos_ground_state_beyond_64_vectors(gpu, orc):
+    """Krylov::ground_state at krydim 72: each restart's answer combines 72
+    basis vectors in one rp_lincomb call; the energy against numpy's lowest
+    eigenvalue of the oracle's dense matrix."""
+    L = 10
+    bits, coeff = orc.syk_terms(20, 0.0, seed=11, fp=64)
+    H = gpu.Hamiltonian(L, bits, coeff, fp=64)
+    s = gpu.DeviceState(L, fp=64)
+    s.random(5)
+    e = s.ground_state(H, krydim=72, eps=1e-9)
+    dense = orc.dense_matrix(L, bits, coeff)
+    e0 = np.linalg.eigvalsh(dense)[0]
+    print("krydim 72: E0 %.12f, eigvalsh %.12f" % (e, e0), flush=True)
+    s.close()
+    H.close()
+    assert abs(e - e0) <= 1e-9 * max(1.0, abs(e0))

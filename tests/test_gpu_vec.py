@@ -202,3 +202,26 @@ def test_freed_blocks_are_reused(gpu):
     assert np.array_equal(a, b)
     gpu.check(hip.rp_free(q))
     gpu.check(hip.rp_free(r))
+
+
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+def test_lincomb_beyond_64_vectors(gpu, dtype):
+    """rp_lincomb takes any number of vectors (16 per sweep): k = 80, the
+    Lanczos ground state's combination at a Krylov dimension of 80."""
+    import reapers_amd as ra
+    rng = np.random.default_rng(80)
+    n, k = 3001, 80
+    p = gpu.RP_C64 if dtype == np.complex64 else gpu.RP_C128
+    Vh = [_rand(rng, n, dtype) for _ in range(k)]
+    Vd = [DeviceBuffer.from_array(v) for v in Vh]
+    ptrs = (ctypes.c_void_p * k)(*[b.ptr.value for b in Vd])
+    c = rng.standard_normal(k) + 1j * rng.standard_normal(k)
+    cc = np.empty(2 * k); cc[0::2] = c.real; cc[1::2] = c.imag
+    y = _rand(rng, n, dtype)
+    dy = DeviceBuffer.from_array(y)
+    ra.check(gpu.hip().rp_lincomb(p, dy.ptr, ptrs, k, cc.ctypes.data_as(ctypes.c_void_p), n))
+    yref = y.astype(complex) + sum(c[i] * Vh[i].astype(complex) for i in range(k))
+    tol = 2e-6 * (1 + np.abs(c).sum()) if dtype == np.complex64 else 1e-12 * (1 + np.abs(c).sum())
+    assert np.abs(dy.to_array(dtype, n) - yref).max() <= tol
+    for b in Vd + [dy]:
+        b.free()
