@@ -56,6 +56,7 @@ struct rp_terms {
     int acc64 = 1;           // accumulate in FP64
     int group_acc = 0;       // FP32 within a group, FP64 across groups (complex<float>)
     uint64_t nterms = 0, nflips = 0, ngroups = 0, nsubs = 0;
+    rp::TileMap tmap{};               // output-tile order of the tiled kernels
     void *d_groups = nullptr, *d_subs = nullptr, *d_terms = nullptr, *d_flat = nullptr;
     void *d_cgroups = nullptr, *d_csubs = nullptr, *d_cterms = nullptr;   // compact stream
     void *d_split = nullptr;          // per-slice partial sums of the split small kernel
@@ -522,11 +523,25 @@ __device__ __forceinline__ uint32_t xcd_tile_index()
     return (nb & 7u) == 0 ? (b & 7u) * (nb >> 3) + (b >> 3) : b;
 }
 
+// the workgroup's output tile under the map tm (rp_tables.h): index bits
+// 0..p-1 = position within the XCD's resident workgroups, p..p+2 = XCD,
+// above = batch; the grid has 2^n >= 2^(p+3) workgroups
+__device__ __forceinline__ uint32_t tile_index(const TileMap &tm)
+{
+    if (!tm.n) return xcd_tile_index();
+    const uint32_t b = blockIdx.x, p = (uint32_t)tm.p, j = b >> 3;
+    const uint32_t idx = ((j >> p) << (p + 3)) | ((b & 7u) << p) | (j & ((1u << p) - 1));
+    uint32_t t = 0;
+    for (int i = 0; i < tm.n; i++)
+        if (idx >> i & 1u) t ^= tm.col[i];
+    return t;
+}
+
 template <typename R, typename A, int TB, int NT, int D, bool GA>
 __global__ void __launch_bounds__(NT, (apply_waves_per_eu<R, TB, NT, D>()))
 k_apply_tiled(const typename CxA<R>::T *__restrict__ in, typename CxA<R>::T *__restrict__ out,
               const GroupRuns *__restrict__ groups, int ngroups, const SubRec<A> *__restrict__ subs,
-              const TermRec<A> *__restrict__ terms, int accumulate)
+              const TermRec<A> *__restrict__ terms, int accumulate, TileMap tm)
 {
     using T = typename CxA<R>::T;
     constexpr int TILE = 1 << TB;
@@ -540,7 +555,7 @@ k_apply_tiled(const typename CxA<R>::T *__restrict__ in, typename CxA<R>::T *__r
     __shared__ T tile3[D > 3 ? TILE : 1];
     T wv[TILE / NT];
     apply_tile<R, A, TB, NT, D, GA>(in, out, groups, ngroups, subs, terms, accumulate, tile0, tile1,
-                                    tile2, tile3, xcd_tile_index() << TB, wv);
+                                    tile2, tile3, tile_index(tm) << TB, wv);
 }
 
 // ---------------------------------------------------------------------------
@@ -807,11 +822,11 @@ template <int TB, int NT>
 __global__ void __launch_bounds__(NT, (apply_waves_per_eu<float, TB, NT, 2>()))
 k_apply_stream(const float2 *__restrict__ in, float2 *__restrict__ out,
                const GroupC *__restrict__ groups, int ngroups, const SubC *__restrict__ subs,
-               const TermC *__restrict__ terms, int accumulate)
+               const TermC *__restrict__ terms, int accumulate, TileMap tm)
 {
     __shared__ float2 tile0[1 << TB];
     __shared__ float2 tile1[1 << TB];
-    stream_tile<TB, NT>(in, out, groups, ngroups, subs, terms, accumulate, xcd_tile_index() << TB,
+    stream_tile<TB, NT>(in, out, groups, ngroups, subs, terms, accumulate, tile_index(tm) << TB,
                         tile0, tile1);
 }
 
@@ -858,7 +873,7 @@ __global__ void __launch_bounds__(NT, (apply_waves_per_eu<R, TB, NT, D>()))
 k_apply_tiled_gs(const typename CxA<R>::T *__restrict__ in, typename CxA<R>::T *__restrict__ out,
                  const GroupRuns *__restrict__ groups, int ngroups,
                  const SubRec<A> *__restrict__ subs, const TermRec<A> *__restrict__ terms,
-                 VecPtrs64 V, int k, double *__restrict__ part, int width)
+                 VecPtrs64 V, int k, double *__restrict__ part, int width, TileMap tm)
 {
     using T = typename CxA<R>::T;
     constexpr int TILE = 1 << TB;
@@ -870,7 +885,7 @@ k_apply_tiled_gs(const typename CxA<R>::T *__restrict__ in, typename CxA<R>::T *
     __shared__ T tile3[D > 3 ? TILE : 1];
     __shared__ double sh[(NT / 64 + 1) * NV];
     T wv[RP];
-    const uint32_t t0 = xcd_tile_index() << TB;
+    const uint32_t t0 = tile_index(tm) << TB;
     apply_tile<R, A, TB, NT, D, GA>(in, out, groups, ngroups, subs, terms, 0, tile0, tile1, tile2,
                                     tile3, t0, wv);
     const uint32_t tid = threadIdx.x;
@@ -1004,12 +1019,27 @@ static bool compact_stream(const Tables<double> &tab, std::vector<GroupC> &cg,
     return true;
 }
 
+// log2 of the tiled kernel's resident workgroups per XCD (32 CUs) at the
+// occupancy apply_waves_per_eu gives it: the p of its output-tile map
+static int xcd_resident_bits(int prec, int tb, int nt, int depth)
+{
+    if (!tb) return 0;
+    const int lds = depth * (1 << tb) * (int)elem_bytes(prec);
+    int wg = std::min((160 * 1024) / lds, (nt == 1024 ? 2048 : 1024) / nt);
+    wg = std::max(wg, 1);
+    int p = 5;
+    while ((2 << p) <= 32 * wg) p++;
+    return p;
+}
+
 template <typename R, typename A>
 static int upload_impl(rp_terms *h, const uint64_t *bits, const double *coeff, uint64_t n)
 {
     Tables<A> tab;
-    int st = build_tables<R, A>(h->len, h->tile_bits, h->threads, bits, coeff, n, tab);
+    int st = build_tables<R, A>(h->len, h->tile_bits, h->threads, bits, coeff, n, tab,
+                                xcd_resident_bits(h->prec, h->tile_bits, h->threads, h->depth));
     if (st) return st;
+    h->tmap = tab.tilemap;
     auto &groups = tab.groups;
     auto &subs = tab.subs;
     auto &terms = tab.terms;
@@ -1089,21 +1119,21 @@ static int launch_tiled(rp_terms *h, void *res, const void *vec, int accumulate)
         if (h->group_acc && h->d_csubs) {
             RP_LAUNCH(RP_KERNEL_APPLY, (k_apply_stream<TB, NT>), dim3((unsigned)grid), dim3(NT), 0,
                       (const T *)vec, (T *)res, (const GroupC *)h->d_cgroups, (int)h->ngroups,
-                      (const SubC *)h->d_csubs, (const TermC *)h->d_cterms, accumulate);
+                      (const SubC *)h->d_csubs, (const TermC *)h->d_cterms, accumulate, h->tmap);
             return RP_OK;
         }
         if (h->group_acc) {
             RP_LAUNCH(RP_KERNEL_APPLY, (k_apply_tiled<R, A, TB, NT, D, true>), dim3((unsigned)grid),
                       dim3(NT), 0, (const T *)vec, (T *)res, (const GroupRuns *)h->d_groups,
                       (int)h->ngroups, (const SubRec<A> *)h->d_subs,
-                      (const TermRec<A> *)h->d_terms, accumulate);
+                      (const TermRec<A> *)h->d_terms, accumulate, h->tmap);
             return RP_OK;
         }
     }
     RP_LAUNCH(RP_KERNEL_APPLY, (k_apply_tiled<R, A, TB, NT, D, false>), dim3((unsigned)grid),
               dim3(NT), 0, (const T *)vec, (T *)res, (const GroupRuns *)h->d_groups,
               (int)h->ngroups, (const SubRec<A> *)h->d_subs, (const TermRec<A> *)h->d_terms,
-              accumulate);
+              accumulate, h->tmap);
     return RP_OK;
 }
 
@@ -1181,12 +1211,12 @@ int apply_dots_launch(rp_terms *h, void *res, const void *vec, const void *const
         RP_LAUNCH(RP_KERNEL_APPLY, (k_apply_tiled_gs<float, double, 12, 512, 2, true>),
                   dim3((unsigned)grid), dim3(512), 0, (const T *)vec, (T *)res,
                   (const GroupRuns *)h->d_groups, (int)h->ngroups, (const SubRec<double> *)h->d_subs,
-                  (const TermRec<double> *)h->d_terms, P, k, part, width);
+                  (const TermRec<double> *)h->d_terms, P, k, part, width, h->tmap);
     else
         RP_LAUNCH(RP_KERNEL_APPLY, (k_apply_tiled_gs<float, double, 12, 512, 2, false>),
                   dim3((unsigned)grid), dim3(512), 0, (const T *)vec, (T *)res,
                   (const GroupRuns *)h->d_groups, (int)h->ngroups, (const SubRec<double> *)h->d_subs,
-                  (const TermRec<double> *)h->d_terms, P, k, part, width);
+                  (const TermRec<double> *)h->d_terms, P, k, part, width, h->tmap);
     *rows = grid;
     return RP_OK;
 }
