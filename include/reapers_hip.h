@@ -305,9 +305,20 @@ typedef struct rp_graph *rp_graph_t;
 /* force the apply's tile bits b: the apply is instantiated for tiles of
  * 32 KiB and 64 KiB, b = log2(32 KiB / (nrhs * element bytes)) or b + 1 */
 #define RP_BATCH_TILE_BITS(b) ((b) << 8)
+/* one problem of at least 20 qubits: the apply gathers each state into a
+ * plane of its own, applies the single-state kernel (rp_apply_ops, the
+ * problem's terms uploaded with its own autotuner) to every plane and
+ * scatters the results back (2 * nrhs * 2^len * S bytes of scratch).  Taken
+ * without this flag when RP_BATCH_AUTOTUNE times it faster than the batched
+ * kernel, or from 22 qubits without the autotuner; RP_BATCH_TILE_BITS keeps
+ * the batched kernel */
+#define RP_BATCH_PLANES 8
 typedef struct {
     int prec, nprob, nrhs, kmax;
-    int tile_bits, threads, grid;   /* apply tiling and persistent grid */
+    int tile_bits, threads, grid;   /* apply tiling and persistent grid (with
+                                       planes: the single-state kernel's tiling
+                                       and workgroups per plane) */
+    int planes;                     /* 1: the plane apply (RP_BATCH_PLANES) */
     uint64_t nbasis;                /* sum of 2^len[p] */
     uint64_t vec_bytes;             /* bytes of one batch vector */
     uint64_t nterms, ngroups;       /* over all problems */
@@ -328,12 +339,16 @@ int rp_batch_set_basis(rp_batch_t b, void *const *V, int n);
 int rp_batch_apply(rp_batch_t b, void *res, const void *vec, int accumulate);
 /* V_i /= ||V_i|| per state; nrm_dev[p * nrhs + s] = the norm (device) */
 int rp_batch_normalize(rp_batch_t b, int i, double *nrm_dev);
-/* Arnoldi step j: V_{j+1} = A V_j, two Gram-Schmidt passes (FP64 dots),
+/* Arnoldi step j: V_{j+1} = A V_j, one Gram-Schmidt pass (FP64 dots) and a
+ * second for the problems with a state whose first-pass norm is below
+ * eta^2 / 2 = 1/4 (the reference's selective criterion from its initial
+ * norms 1 and eta / 2, decided on the device),
  * H(0..j+1, j) into H_dev (per state a column-major (kmax+1) x kmax complex
  * double matrix, states consecutive), V_{j+1} normalized; kernels only.
- * H(j+1, j) is real: its real part is the norm after both passes, its
- * imaginary slot holds the norm after the first pass (for the reference's
- * breakdown test, norm < norm_1 / sqrt(2): inc/algo.h:79-106). */
+ * H(j+1, j) is real: its real part is the norm after the last pass (the
+ * first, when the second did not run), its imaginary slot holds the norm
+ * after the first pass (for the reference's breakdown test, norm < norm_1 /
+ * sqrt(2): inc/algo.h:79-106). */
 int rp_batch_arnoldi_step(rp_batch_t b, int j, double *H_dev);
 /* y += sum_{i<k} c[rhs][i] V_i, c complex double (device), cstride doubles per state */
 int rp_batch_lincomb(rp_batch_t b, void *y, int k, const double *c_dev, int cstride);

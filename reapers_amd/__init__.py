@@ -59,8 +59,8 @@ class EvolveStats(_c.Structure):
 class BatchInfo(_c.Structure):
     _fields_ = [("prec", _c.c_int), ("nprob", _c.c_int), ("nrhs", _c.c_int), ("kmax", _c.c_int),
                 ("tile_bits", _c.c_int), ("threads", _c.c_int), ("grid", _c.c_int),
-                ("nbasis", _u64), ("vec_bytes", _u64), ("nterms", _u64), ("ngroups", _u64),
-                ("bytes_per_apply", _u64)]
+                ("planes", _c.c_int), ("nbasis", _u64), ("vec_bytes", _u64), ("nterms", _u64),
+                ("ngroups", _u64), ("bytes_per_apply", _u64)]
 
 
 _hip = None
@@ -794,7 +794,7 @@ class Batch:
     states each in the block-state layout."""
 
     def __init__(self, lens, terms, nrhs=4, kmax=20, fp=32, autotune=True, tile_bits=0,
-                 acc_group=False, wide=False):
+                 acc_group=False, wide=False, planes=False):
         self.lens = [int(x) for x in lens]
         self.nprob = len(self.lens)
         self.nrhs = nrhs
@@ -808,7 +808,8 @@ class Batch:
         self._h = host().rph_batch_new(fp, self.nprob, _ptr(la), _ptr(bits), _ptr(co), _ptr(toff),
                                        nrhs, kmax, ((tile_bits << 8) if tile_bits
                                                     else (1 if autotune else 0))
-                                       | (2 if acc_group else 0) | (4 if wide else 0))
+                                       | (2 if acc_group else 0) | (4 if wide else 0)
+                                       | (8 if planes else 0))
         if not self._h:
             raise RpError(-1, host().rph_last_error().decode())
         self.info = BatchInfo()

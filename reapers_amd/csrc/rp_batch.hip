@@ -18,9 +18,12 @@
 // buffered, every sub of the group applied from LDS with the sign split into
 // lane, row and workgroup parts), with NS accumulators per amplitude.
 //
-// Arnoldi step.  V_{j+1} = A V_j, two classical Gram-Schmidt passes against
-// V_0..V_j with FP64 dot products (fixed count, so that the step is a fixed
-// sequence of kernels that a HIP graph can capture), H(0..j, j) and
+// Arnoldi step.  V_{j+1} = A V_j, one classical Gram-Schmidt pass against
+// V_0..V_j with FP64 dot products and a second one for the problems with a
+// state whose first-pass norm asks for it (the reference's selective
+// criterion, decided on the device: the step stays a fixed sequence of
+// kernels that a HIP graph can capture, the second pass's kernels return at
+// once for the other problems), H(0..j, j) and
 // H(j+1, j) = ||V_{j+1}|| written to a device matrix per right-hand side,
 // and V_{j+1} scaled by 1/||V_{j+1}|| on the device.  Reductions are two
 // stage (per chunk of 2^cb amplitudes, then per problem in chunk order):
@@ -65,8 +68,15 @@ struct rp_batch {
     void *d_npart = nullptr;           // chunk norm partials: ns double
     void *d_h = nullptr;               // per rhs: kmax+1 complex double (current pass)
     void *d_inv = nullptr;             // per rhs: 1 / norm
+    void *d_need2 = nullptr;           // per rhs: the Arnoldi step's second pass is needed
     std::vector<void *> V;             // host copy of the basis table
     int grid = 0;                      // persistent apply grid
+    // state planes (one problem of >= kPlanesMinLen qubits): the apply
+    // gathers each state into a plane of its own, applies the single-state
+    // kernel to every plane and scatters the results back
+    int planes = 0;
+    rp_terms_t plane_terms = nullptr;
+    void *d_plane_in = nullptr, *d_plane_out = nullptr;
 };
 
 namespace rp {
@@ -76,6 +86,47 @@ constexpr int kRedT = 256;            // threads of the reduction kernels
 constexpr int kKC = 8;                // basis vectors per sweep of the dot kernel
 
 template <typename R, int NS> struct Elt { typename CxA<R>::T v[NS]; };
+
+constexpr int kPlanesMinLen = 20;     // smallest problem the plane apply takes
+constexpr int kPlaneT = 256;          // threads of the gather / scatter kernels
+
+// plane s of the NS-state block: planes[s * n + m] = in[m].v[s]; nothing
+// when the (only) problem is inactive
+template <typename R, int NS>
+__global__ void __launch_bounds__(kPlaneT)
+k_block_to_planes(const Elt<R, NS> *__restrict__ in, typename CxA<R>::T *__restrict__ planes,
+                  uint64_t n, const int *__restrict__ active)
+{
+    if (!active[0]) return;
+    const uint64_t m = (uint64_t)blockIdx.x * kPlaneT + threadIdx.x;
+    if (m >= n) return;
+    const Elt<R, NS> e = in[m];
+#pragma unroll
+    for (int q = 0; q < NS; q++) planes[q * n + m] = e.v[q];
+}
+
+// out[m].v[s] (+)= planes[s * n + m]
+template <typename R, int NS>
+__global__ void __launch_bounds__(kPlaneT)
+k_planes_to_block(const typename CxA<R>::T *__restrict__ planes, Elt<R, NS> *__restrict__ out,
+                  uint64_t n, int accumulate, const int *__restrict__ active)
+{
+    if (!active[0]) return;
+    const uint64_t m = (uint64_t)blockIdx.x * kPlaneT + threadIdx.x;
+    if (m >= n) return;
+    Elt<R, NS> e;
+#pragma unroll
+    for (int q = 0; q < NS; q++) e.v[q] = planes[q * n + m];
+    if (accumulate) {
+        const Elt<R, NS> o = out[m];
+#pragma unroll
+        for (int q = 0; q < NS; q++) {
+            e.v[q].x += o.v[q].x;
+            e.v[q].y += o.v[q].y;
+        }
+    }
+    out[m] = e;
+}
 
 template <typename R, typename A, int TB, int NT, int NS>
 constexpr int batch_waves_per_eu()
@@ -329,18 +380,29 @@ __device__ inline double block_sum_by_state(double v, int ns, double *red)
     return a;
 }
 
+// a Gram-Schmidt pass after the first runs for problem p when one of its
+// states asked for it (need: per rhs flags, or null: always)
+__device__ inline bool pass_needed(const int *__restrict__ need, uint32_t p, int ns)
+{
+    if (!need) return true;
+    for (int s = 0; s < ns; s++)
+        if (need[(size_t)p * ns + s]) return true;
+    return false;
+}
+
 // part[c][i][s] = sum over the chunk of conj(V_i) w   (i < k, complex double)
 template <typename R>
 __global__ void __launch_bounds__(kRedT)
 k_batch_dots(const void *const *__restrict__ Vt, int k, const typename CxA<R>::T *__restrict__ w,
              const ProbDesc *__restrict__ probs, const uint32_t *__restrict__ chunk_prob,
-             const int *__restrict__ active, int ns, int cb, double *__restrict__ part)
+             const int *__restrict__ active, int ns, int cb, double *__restrict__ part,
+             const int *__restrict__ need)
 {
     using T = typename CxA<R>::T;
     __shared__ double red[kRedT];
     const uint32_t c = blockIdx.x;
     const uint32_t p = chunk_prob[c];
-    if (!active[p]) return;
+    if (!active[p] || !pass_needed(need, p, ns)) return;
     const ProbDesc P = probs[p];
     const uint64_t base = (P.off + ((uint64_t)(c - P.chunk0) << cb)) * ns;
     const uint32_t nel = (1u << cb) * ns;
@@ -397,11 +459,12 @@ __device__ inline double block_chunk_sum(const double *__restrict__ src, size_t 
 __global__ void __launch_bounds__(kRedT)
 k_batch_reduce_dots(const double *__restrict__ part, int k, const ProbDesc *__restrict__ probs,
                     const int *__restrict__ active, int ns, double *__restrict__ h, int hstride,
-                    double *__restrict__ H, int j, int kmax, int first_pass)
+                    double *__restrict__ H, int j, int kmax, int first_pass,
+                    const int *__restrict__ need)
 {
     __shared__ double red[kRedT];
     const int p = blockIdx.x, e = blockIdx.y;
-    if (!active[p] || e >= k * ns) return;
+    if (!active[p] || e >= k * ns || !pass_needed(need, p, ns)) return;
     const ProbDesc P = probs[p];
     const int i = e / ns, s = e % ns;
     const size_t stride = 2 * (size_t)ns * 65;
@@ -426,14 +489,14 @@ __global__ void __launch_bounds__(kRedT)
 k_batch_update(const void *const *__restrict__ Vt, int k, typename CxA<R>::T *__restrict__ w,
                const ProbDesc *__restrict__ probs, const uint32_t *__restrict__ chunk_prob,
                const int *__restrict__ active, int ns, int cb, const double *__restrict__ h,
-               int hstride, double *__restrict__ npart)
+               int hstride, double *__restrict__ npart, const int *__restrict__ need)
 {
     using T = typename CxA<R>::T;
     __shared__ double red[kRedT];
     __shared__ double hs[2 * 65 * 4];
     const uint32_t c = blockIdx.x;
     const uint32_t p = chunk_prob[c];
-    if (!active[p]) return;
+    if (!active[p] || !pass_needed(need, p, ns)) return;
     const ProbDesc P = probs[p];
     for (int e = threadIdx.x; e < k * ns; e += kRedT) {
         const int i = e / ns, s = e % ns;
@@ -491,13 +554,17 @@ k_batch_norm_part(const typename CxA<R>::T *__restrict__ v, const ProbDesc *__re
 // per rhs: nrm = sqrt(sum of the problem's chunk partials); inv = 1/nrm
 // (0 if nrm == 0); optionally H(j+1, j) = nrm and nrm_out[rhs] = nrm
 // im_only: write the norm to Im H(j+1, j) only (the norm after the first
-// Gram-Schmidt pass, kept for the breakdown test) and leave inv alone
+// Gram-Schmidt pass, kept for the breakdown test) and leave inv alone; with
+// need, also whether this state needs a second pass: the reference's
+// selective criterion (inc/algo.h:79-101) starts from old norm 1 and norm
+// eta/2, so after the first pass a second one runs iff 0 < norm < eta^2/2
 // ||v|| of every rhs from the chunk partials npart[c][s]: block rhs
 __global__ void __launch_bounds__(kRedT)
 k_batch_norm_final(const double *__restrict__ npart, const ProbDesc *__restrict__ probs,
                    const int *__restrict__ active, int nprob, int ns,
                    double *__restrict__ inv, double *__restrict__ nrm_out,
-                   double *__restrict__ H, int j, int kmax, int im_only)
+                   double *__restrict__ H, int j, int kmax, int im_only,
+                   int *__restrict__ need)
 {
     __shared__ double red[kRedT];
     const int rhs = blockIdx.x;
@@ -512,6 +579,7 @@ k_batch_norm_final(const double *__restrict__ npart, const ProbDesc *__restrict_
                     : nullptr;
     if (im_only) {
         if (Hij) Hij[1] = n;
+        if (need) need[rhs] = n > 0 && n < 0.25;
         return;
     }
     inv[rhs] = n > 0 ? 1.0 / n : 0.0;
@@ -594,12 +662,72 @@ int batch_alloc(void **p, size_t bytes)
     return RP_OK;
 }
 
+void release_planes(rp_batch *b)
+{
+    if (b->plane_terms) (void)rp_terms_free(b->plane_terms);
+    for (void *p : {b->d_plane_in, b->d_plane_out})
+        if (p) (void)hipFree(p);
+    b->plane_terms = nullptr;
+    b->d_plane_in = b->d_plane_out = nullptr;
+    b->planes = 0;
+}
+
+// the plane apply's resources: the single-state term handle (its own
+// tiling autotuner when autotune, group accumulation as the batch) and two
+// NS-plane scratch vectors
+int setup_planes(rp_batch *b, const uint64_t *bits, const double *coeff, uint64_t nterms,
+                 int autotune)
+{
+    const int fl = (autotune ? RP_TERMS_AUTOTUNE : 0) | (b->group_acc ? RP_TERMS_ACC_GROUP : 0);
+    int st = rp_terms_upload_ex(&b->plane_terms, b->prec, b->len[0], bits, coeff, nterms, fl);
+    const size_t bytes = ((size_t)b->ns * elem_bytes(b->prec)) << b->len[0];
+    if (!st) st = batch_alloc(&b->d_plane_in, bytes);
+    if (!st) st = batch_alloc(&b->d_plane_out, bytes);
+    if (st) release_planes(b);
+    return st;
+}
+
+int launch_apply(rp_batch *b, void *res, const void *vec, int accumulate);
+
+// average time of 3 batched applies (after a warm-up) on zeroed scratch
+// vectors with the plane apply on or off
+int time_batch_apply(rp_batch *b, int planes, float *ms)
+{
+    Context &c = ctx();
+    void *x = nullptr, *y = nullptr;
+    const size_t bytes = b->nbasis * b->ns * elem_bytes(b->prec);
+    int st = batch_alloc(&x, bytes);
+    if (!st) st = batch_alloc(&y, bytes);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (!st && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+        st = set_error(RP_ERR_RUNTIME, "hipEventCreate failed");
+    const int keep = b->planes;
+    b->planes = planes;
+    if (!st && hipMemsetAsync(x, 0, bytes, c.stream) != hipSuccess)
+        st = set_error(RP_ERR_RUNTIME, "hipMemsetAsync failed");
+    if (!st) st = launch_apply(b, y, x, 0);
+    if (!st && hipEventRecord(e0, c.stream) != hipSuccess)
+        st = set_error(RP_ERR_RUNTIME, "hipEventRecord failed");
+    for (int r = 0; r < 3 && !st; r++) st = launch_apply(b, y, x, 0);
+    if (!st && (hipEventRecord(e1, c.stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(ms, e0, e1) != hipSuccess))
+        st = set_error(RP_ERR_RUNTIME, "timing the batched apply failed");
+    b->planes = keep;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (x) (void)hipFree(x);
+    if (y) (void)hipFree(y);
+    return st;
+}
+
 void batch_release(rp_batch *b)
 {
     if (!b) return;
     for (void *p : {b->d_probs, b->d_tile_prob, b->d_chunk_prob, b->d_groups, b->d_subs,
-                    b->d_terms, b->d_active, b->d_V, b->d_part, b->d_npart, b->d_h, b->d_inv})
+                    b->d_terms, b->d_active, b->d_V, b->d_part, b->d_npart, b->d_h, b->d_inv,
+                    b->d_need2})
         if (p) (void)hipFree(p);
+    release_planes(b);
     delete b;
 }
 
@@ -650,8 +778,41 @@ int launch_apply_ns(rp_batch *b, void *res, const void *vec, int accumulate)
                      b->tile_bits, NS);
 }
 
+template <typename R, int NS>
+int launch_planes_ns(rp_batch *b, void *res, const void *vec, int accumulate)
+{
+    using T = typename CxA<R>::T;
+    const uint64_t n = 1ull << b->len[0];
+    const dim3 grid((unsigned)((n + kPlaneT - 1) / kPlaneT));
+    const int *act = (const int *)b->d_active;
+    RP_LAUNCH(RP_KERNEL_APPLY, (k_block_to_planes<R, NS>), grid, dim3(kPlaneT), 0,
+              (const Elt<R, NS> *)vec, (T *)b->d_plane_in, n, act);
+    for (int q = 0; q < NS; q++) {
+        int st = rp_apply_ops(b->plane_terms, (T *)b->d_plane_out + q * n,
+                              (const T *)b->d_plane_in + q * n, 0);
+        if (st) return st;
+    }
+    RP_LAUNCH(RP_KERNEL_APPLY, (k_planes_to_block<R, NS>), grid, dim3(kPlaneT), 0,
+              (const T *)b->d_plane_out, (Elt<R, NS> *)res, n, accumulate, act);
+    return RP_OK;
+}
+
+template <typename R>
+int launch_planes(rp_batch *b, void *res, const void *vec, int accumulate)
+{
+    switch (b->ns) {
+    case 1: return launch_planes_ns<R, 1>(b, res, vec, accumulate);
+    case 2: return launch_planes_ns<R, 2>(b, res, vec, accumulate);
+    case 4: return launch_planes_ns<R, 4>(b, res, vec, accumulate);
+    }
+    return set_error(RP_ERR_INVALID_ARGUMENT, "batched apply: nrhs must be 1, 2 or 4");
+}
+
 int launch_apply(rp_batch *b, void *res, const void *vec, int accumulate)
 {
+    if (b->planes)
+        return b->prec == RP_C64 ? launch_planes<float>(b, res, vec, accumulate)
+                                 : launch_planes<double>(b, res, vec, accumulate);
     if (b->prec == RP_C64) {
         switch (b->ns) {
         case 1: return launch_apply_ns<float, double, 1>(b, res, vec, accumulate);
@@ -737,11 +898,6 @@ int build_batch_tables(rp_batch *b, int tb, int nt, const uint64_t *bits, const 
     return RP_OK;
 }
 
-int batch_dims_check(rp_batch *b)
-{
-    return b ? RP_OK : set_error(RP_ERR_INVALID_ARGUMENT, "null batch handle");
-}
-
 }  // namespace
 }  // namespace rp
 
@@ -761,8 +917,11 @@ int rp_batch_create(rp_batch_t *out, int prec, int nprob, const int *len, const 
     RP_REQUIRE(len && term_off && (term_off[nprob] == 0 || (bits && coeff)), "null argument");
     RP_REQUIRE(nrhs == 1 || nrhs == 2 || nrhs == 4, "nrhs must be 1, 2 or 4");
     RP_REQUIRE(kmax >= 2 && kmax <= 64, "kmax must be in [2, 64]");
-    RP_REQUIRE((flags & ~(RP_BATCH_AUTOTUNE | RP_BATCH_ACC_GROUP | RP_BATCH_WIDE | 0xff00)) == 0,
+    RP_REQUIRE((flags & ~(RP_BATCH_AUTOTUNE | RP_BATCH_ACC_GROUP | RP_BATCH_WIDE |
+                          RP_BATCH_PLANES | 0xff00)) == 0,
                "unknown flags");
+    RP_REQUIRE(!(flags & RP_BATCH_PLANES) || (nprob == 1 && len[0] >= kPlanesMinLen),
+               "RP_BATCH_PLANES needs one problem of at least 20 qubits");
     const int forced_tb = (flags >> 8) & 0xff;
     RP_REQUIRE(forced_tb == 0 || forced_tb == tile_bits_for(prec, nrhs, 0) ||
                    forced_tb == tile_bits_for(prec, nrhs, 1),
@@ -808,7 +967,8 @@ int rp_batch_create(rp_batch_t *out, int prec, int nprob, const int *len, const 
         (st = batch_alloc(&b->d_part, (size_t)chunks * nrhs * 65 * 2 * sizeof(double))) ||
         (st = batch_alloc(&b->d_npart, (size_t)chunks * nrhs * sizeof(double))) ||
         (st = batch_alloc(&b->d_h, nr * 65 * 2 * sizeof(double))) ||
-        (st = batch_alloc(&b->d_inv, nr * sizeof(double)))) {
+        (st = batch_alloc(&b->d_inv, nr * sizeof(double))) ||
+        (st = batch_alloc(&b->d_need2, nr * sizeof(int)))) {
         batch_release(b);
         return st;
     }
@@ -881,6 +1041,29 @@ int rp_batch_create(rp_batch_t *out, int prec, int nprob, const int *len, const 
             return st;
         }
     }
+    // one problem of kPlanesMinLen qubits or more: the plane apply, forced,
+    // or kept when the autotuner times it faster than the batched kernel,
+    // or from 22 qubits without the autotuner
+    const bool autot = (flags & RP_BATCH_AUTOTUNE) && !forced_tb;
+    if (nprob == 1 && len[0] >= kPlanesMinLen && !forced_tb &&
+        ((flags & RP_BATCH_PLANES) || autot || len[0] >= 22)) {
+        const int64_t t0 = term_off[0];
+        if ((st = setup_planes(b, bits + t0, coeff + 2 * t0, (uint64_t)(term_off[1] - t0),
+                               autot))) {
+            batch_release(b);
+            return st;
+        }
+        b->planes = 1;
+        if (autot && !(flags & RP_BATCH_PLANES)) {
+            float ms[2] = {0, 0};
+            if ((st = time_batch_apply(b, 0, &ms[0])) || (st = time_batch_apply(b, 1, &ms[1]))) {
+                batch_release(b);
+                return st;
+            }
+            if (ms[0] <= ms[1]) release_planes(b);
+            else b->planes = 1;
+        }
+    }
     *out = b;
     return RP_OK;
 }
@@ -905,6 +1088,7 @@ int rp_batch_info(rp_batch_t b, rp_batch_info_t *info)
     info->tile_bits = b->tile_bits;
     info->threads = b->threads;
     info->grid = b->grid;
+    info->planes = b->planes;
     info->nbasis = b->nbasis;
     info->vec_bytes = b->nbasis * b->ns * elem_bytes(b->prec);
     info->nterms = b->nterms;
@@ -915,6 +1099,16 @@ int rp_batch_info(rp_batch_t b, rp_batch_info_t *info)
         info->bytes_per_apply += (uint64_t)(P.g1 - P.g0 + 1) << P.len;
     }
     info->bytes_per_apply *= b->ns * elem_bytes(b->prec);
+    if (b->planes) {   // the single-state kernel's layout
+        rp_terms_info_t ti;
+        int st = rp_terms_info(b->plane_terms, &ti);
+        if (st) return st;
+        info->tile_bits = ti.tile_bits;
+        info->threads = ti.threads;
+        info->grid = (int)((1ull << b->len[0]) >> (ti.tile_bits ? ti.tile_bits : 8));
+        info->ngroups = ti.ngroups;
+        info->bytes_per_apply = ti.bytes_per_apply * b->ns;
+    }
     return RP_OK;
 }
 
@@ -992,7 +1186,7 @@ int rp_batch_normalize(rp_batch_t b, int i, double *nrm_out)
     }
     RP_LAUNCH(RP_KERNEL_GS, k_batch_norm_final, dim3(nr), dim3(kRedT), 0,
               (const double *)b->d_npart, P, act, b->nprob, b->ns, (double *)b->d_inv, nrm_out,
-              (double *)nullptr, 0, b->kmax, 0);
+              (double *)nullptr, 0, b->kmax, 0, (int *)nullptr);
     if (b->prec == RP_C64) {
         RP_LAUNCH(RP_KERNEL_GS, k_batch_scale<float>, gc, dim3(kRedT), 0, (float2 *)b->V[i], P,
                   cp, act, b->ns, b->chunk_bits, (const double *)b->d_inv);
@@ -1021,34 +1215,43 @@ int rp_batch_arnoldi_step(rp_batch_t b, int j, double *H)
     const void *const *Vt = (const void *const *)b->d_V;
     const dim3 gc((unsigned)b->total_chunks);
     const int k = j + 1, hstride = 65 * 2, nr = b->nprob * b->ns;
+    // the first pass always; the second only for the problems with a state
+    // whose first-pass norm asks for it (the flags are set on the device, so
+    // the step stays one fixed kernel sequence for graph capture).  A problem
+    // that skips it keeps the first pass's norm partials, so the second
+    // norm_final reads the first pass's norm.
+    int *need2 = (int *)b->d_need2;
     for (int pass = 0; pass < 2; pass++) {
+        const int *need = pass == 0 ? nullptr : need2;
         if (b->prec == RP_C64) {
             RP_LAUNCH(RP_KERNEL_GS, k_batch_dots<float>, gc, dim3(kRedT), 0, Vt, k,
                       (const float2 *)b->V[j + 1], P, cp, act, b->ns, b->chunk_bits,
-                      (double *)b->d_part);
+                      (double *)b->d_part, need);
         } else {
             RP_LAUNCH(RP_KERNEL_GS, k_batch_dots<double>, gc, dim3(kRedT), 0, Vt, k,
                       (const double2 *)b->V[j + 1], P, cp, act, b->ns, b->chunk_bits,
-                      (double *)b->d_part);
+                      (double *)b->d_part, need);
         }
         RP_LAUNCH(RP_KERNEL_GS, k_batch_reduce_dots, dim3((unsigned)b->nprob, (unsigned)(k * b->ns)),
                   dim3(kRedT), 0,
                   (const double *)b->d_part, k, P, act, b->ns, (double *)b->d_h, hstride, H, j,
-                  b->kmax, pass == 0 ? 1 : 0);
+                  b->kmax, pass == 0 ? 1 : 0, need);
         if (b->prec == RP_C64) {
             RP_LAUNCH(RP_KERNEL_GS, k_batch_update<float>, gc, dim3(kRedT), 0, Vt, k,
                       (float2 *)b->V[j + 1], P, cp, act, b->ns, b->chunk_bits,
-                      (const double *)b->d_h, hstride, (double *)b->d_npart);
+                      (const double *)b->d_h, hstride, (double *)b->d_npart, need);
         } else {
             RP_LAUNCH(RP_KERNEL_GS, k_batch_update<double>, gc, dim3(kRedT), 0, Vt, k,
                       (double2 *)b->V[j + 1], P, cp, act, b->ns, b->chunk_bits,
-                      (const double *)b->d_h, hstride, (double *)b->d_npart);
+                      (const double *)b->d_h, hstride, (double *)b->d_npart, need);
         }
-        // after the first pass: its norm to Im H(j+1, j); after the second:
-        // the norm to Re H(j+1, j) and its inverse for the normalization
+        // after the first pass: its norm to Im H(j+1, j) and the second-pass
+        // flags; after the second: the norm to Re H(j+1, j) and its inverse
+        // for the normalization
         RP_LAUNCH(RP_KERNEL_GS, k_batch_norm_final, dim3(nr), dim3(kRedT), 0,
                   (const double *)b->d_npart, P, act, b->nprob, b->ns, (double *)b->d_inv,
-                  (double *)nullptr, H, j, b->kmax, pass == 0 ? 1 : 0);
+                  (double *)nullptr, H, j, b->kmax, pass == 0 ? 1 : 0,
+                  pass == 0 ? need2 : (int *)nullptr);
     }
     if (b->prec == RP_C64) {
         RP_LAUNCH(RP_KERNEL_GS, k_batch_scale<float>, gc, dim3(kRedT), 0, (float2 *)b->V[j + 1],

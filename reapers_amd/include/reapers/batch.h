@@ -44,7 +44,8 @@ class BatchedStates {
 
 public:
     // hams[p] acts on lens[p] qubits; kmax bounds the Krylov dimension; flags
-    // as rp_batch_create (RP_BATCH_AUTOTUNE, RP_BATCH_ACC_GROUP, RP_BATCH_TILE_BITS(b))
+    // as rp_batch_create (RP_BATCH_AUTOTUNE, RP_BATCH_ACC_GROUP, RP_BATCH_WIDE,
+    // RP_BATCH_PLANES, RP_BATCH_TILE_BITS(b))
     BatchedStates(const std::vector<const HostSumOps<FpType> *> &hams, std::vector<int> lens_,
                   int nrhs, int kmax_, int flags = RP_BATCH_AUTOTUNE)
         : lens(std::move(lens_)), nprob((int)hams.size()), ns(nrhs), kmax(kmax_) {

@@ -191,3 +191,34 @@ def test_batch_group_accumulation_vs_oracle(gpu, orc, nrhs, large):
             worst = max(worst, np.linalg.norm(got[p][:, s] - ref))
     print("acc group nrhs=%d tb=%d: evolve max |err| %.2e" % (nrhs, tb, worst), flush=True)
     assert worst < 1e-5
+
+
+def test_batch_selective_second_pass(gpu, orc):
+    """The batched Arnoldi step's second Gram-Schmidt pass runs only for
+    problems with a state whose first-pass norm is below 1/4 (the reference's
+    selective criterion, inc/algo.h:79-101, decided on the device): a
+    problem scaled to a small norm, whose every step takes the second pass,
+    beside one of ordinary norm, whose steps skip it; both against the
+    oracle's FP64 evolution."""
+    rng = np.random.default_rng(44)
+    lens = [12, 13]
+    b1, c1 = orc.syk_terms(24, 8.0, seed=31, fp=64)
+    b2, c2 = orc.syk_terms(26, 8.0, seed=32, fp=64)
+    terms = [(b1, c1 * 1e-3), (b2, c2)]
+    t = 3.0
+    states = [np.stack([random_state(rng, L) for _ in range(2)], 1) for L in lens]
+    for graph in (True, False):
+        b = gpu.Batch(lens, terms, nrhs=2, kmax=10, fp=64)
+        b.load(states)
+        st, rs = b.evolve(t, m=10, tol=1e-12, graph=graph)
+        got = b.store()
+        b.close()
+        worst = 0.0
+        for p, L in enumerate(lens):
+            for s in range(2):
+                ref, _ = orc.krylov_evolve(L, *terms[p], states[p][:, s], t, 0.0, krydim=20,
+                                           tol=1e-14)
+                worst = max(worst, np.linalg.norm(got[p][:, s] - ref))
+        print("selective second pass (graph=%d): max |err| %.2e, restarts %s" % (
+            graph, worst, rs.max(1).tolist()), flush=True)
+        assert worst < 1e-11
