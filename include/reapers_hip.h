@@ -310,8 +310,8 @@ typedef struct rp_graph *rp_graph_t;
  * problem's terms uploaded with its own autotuner) to every plane and
  * scatters the results back (2 * nrhs * 2^len * S bytes of scratch).  Taken
  * without this flag when RP_BATCH_AUTOTUNE times it faster than the batched
- * kernel, or from 22 qubits without the autotuner; RP_BATCH_TILE_BITS keeps
- * the batched kernel */
+ * kernel, or from 22 qubits without the autotuner; RP_BATCH_TILE_BITS, and a
+ * term list with no nonzero term, keep the batched kernel */
 #define RP_BATCH_PLANES 8
 typedef struct {
     int prec, nprob, nrhs, kmax;

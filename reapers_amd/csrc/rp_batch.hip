@@ -680,6 +680,14 @@ int setup_planes(rp_batch *b, const uint64_t *bits, const double *coeff, uint64_
 {
     const int fl = (autotune ? RP_TERMS_AUTOTUNE : 0) | (b->group_acc ? RP_TERMS_ACC_GROUP : 0);
     int st = rp_terms_upload_ex(&b->plane_terms, b->prec, b->len[0], bits, coeff, nterms, fl);
+    // no term left (H = 0): the single-state apply would clear its output
+    // with a memset, which a captured restart must not hold; the batched
+    // kernel takes it
+    rp_terms_info_t ti;
+    if (!st && !(st = rp_terms_info(b->plane_terms, &ti)) && ti.nterms == 0) {
+        release_planes(b);
+        return RP_OK;
+    }
     const size_t bytes = ((size_t)b->ns * elem_bytes(b->prec)) << b->len[0];
     if (!st) st = batch_alloc(&b->d_plane_in, bytes);
     if (!st) st = batch_alloc(&b->d_plane_out, bytes);
@@ -1053,8 +1061,8 @@ int rp_batch_create(rp_batch_t *out, int prec, int nprob, const int *len, const 
             batch_release(b);
             return st;
         }
-        b->planes = 1;
-        if (autot && !(flags & RP_BATCH_PLANES)) {
+        b->planes = b->plane_terms != nullptr;
+        if (b->planes && autot && !(flags & RP_BATCH_PLANES)) {
             float ms[2] = {0, 0};
             if ((st = time_batch_apply(b, 0, &ms[0])) || (st = time_batch_apply(b, 1, &ms[1]))) {
                 batch_release(b);

@@ -132,3 +132,18 @@ def test_plane_autotune_choice(gpu, orc):
     print("autotuned one-problem batch at %d qubits: planes=%d tiling %d/%d" % (
         L, b.info.planes, b.info.tile_bits, b.info.threads), flush=True)
     b.close()
+
+
+def test_plane_apply_declines_an_empty_term_list(gpu):
+    """H = 0 (its one term has a zero coefficient): the single-state apply
+    would clear its output with a memset, which a captured restart must not
+    hold, so the batch keeps its own kernel; the apply gives zero."""
+    bits = np.array([0], np.uint64)
+    coeff = np.array([0.0 + 0.0j])
+    b = gpu.Batch([L], [(bits, coeff)], nrhs=2, kmax=4, fp=32, autotune=False, planes=True)
+    assert b.info.planes == 0
+    rng = np.random.default_rng(3)
+    b.load([np.stack([random_state(rng, L, np.complex64) for _ in range(2)], 1)])
+    b.apply()
+    assert not np.any(b.store()[0])
+    b.close()
