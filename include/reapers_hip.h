@@ -154,6 +154,9 @@ typedef struct {
     double   apply_passes;  /* the kept tiling's apply time over one streaming pass
                                of the state (half an rp_scale_vec), measured by the
                                autotuner at upload; 0 without RP_TERMS_AUTOTUNE */
+    int      tile_map;      /* tiled kernels: columns of the output-tile map that
+                               orders the groups by XCD class (0: contiguous tiles,
+                               groups by flip mask) */
 } rp_terms_info_t;
 /* the apply kernel family a term handle launches: one thread per amplitude
    (tile_bits 0); tiles with per-sub records; tiles with the compact sub

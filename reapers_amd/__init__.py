@@ -43,7 +43,7 @@ class TermsInfo(_c.Structure):
     _fields_ = [("prec", _c.c_int), ("len", _c.c_int), ("tile_bits", _c.c_int),
                 ("nterms", _u64), ("nflips", _u64), ("ngroups", _u64),
                 ("bytes_per_apply", _u64), ("threads", _c.c_int), ("depth", _c.c_int),
-                ("kernel", _c.c_int), ("apply_passes", _dbl)]
+                ("kernel", _c.c_int), ("apply_passes", _dbl), ("tile_map", _c.c_int)]
 
 
 class EvolveStats(_c.Structure):

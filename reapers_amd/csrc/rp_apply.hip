@@ -1400,6 +1400,7 @@ int rp_terms_info(rp_terms_t h, rp_terms_info_t *info)
                        ? RP_APPLY_STREAM
                        : RP_APPLY_TILED;
     info->bytes_per_apply = (info->ngroups + 1) * (1ull << h->len) * elem_bytes(h->prec);
+    info->tile_map = h->tmap.n;
     return RP_OK;
 }
 
